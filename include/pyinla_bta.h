@@ -1,0 +1,109 @@
+/*
+ * pyinla_bta.h -- C ABI of the MI355X (gfx950) BTA solver library
+ * (libpyinla_bta.so).
+ *
+ * The library replaces the arithmetic that pyINLA's SerinvSolver hands to the
+ * external Serinv package, on block-tridiagonal-arrowhead (BTA) matrices:
+ *
+ *   D  (n, b, b)   diagonal blocks          A_diagonal_blocks
+ *   L  (n-1, b, b) sub-diagonal blocks      A_lower_diagonal_blocks
+ *   B  (n, a, b)   arrow (bottom) blocks    A_arrow_bottom_blocks
+ *   T  (a, a)      arrow tip                A_arrow_tip_block
+ *
+ * (src/pyinla/solvers/structured_solver.py:36-60).  All blocks are FP64,
+ * row-major, contiguous, with leading dimension b (a for T).  a == 0 selects
+ * the BT variant (no arrowhead: B and T may be NULL).
+ *
+ * Conventions
+ *   - Pointers are device pointers on the workspace's device.
+ *   - stream is a hipStream_t passed as void* (NULL = default stream).
+ *   - Every function returns 0 on success, a negative BTA_E_* code on invalid
+ *     arguments, or a positive hipError_t code from the runtime.
+ *   - No function allocates device memory except bta_ws_create.
+ *   - Calls are asynchronous on the stream; results that a host needs
+ *     (logdet, info) are read with bta_ws_status, which synchronises.
+ */
+#ifndef PYINLA_BTA_H
+#define PYINLA_BTA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BTA_OK 0
+#define BTA_E_ARG (-1)      /* invalid shape / pointer / option */
+#define BTA_E_SHAPE (-2)    /* shape does not match the workspace */
+#define BTA_E_TIMEOUT (-3)  /* a bounded in-kernel wait timed out */
+
+typedef struct bta_ws* bta_ws_t;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int bta_version(void);
+
+/* Device workspace for one solver instance of shape (n, b, a).  Holds the
+ * inverse of every 64x64 diagonal tile of the factor, readiness flags, the
+ * log-determinant partials and the scratch used by solve / selected
+ * inversion.  Replaces the buffer ownership of SerinvSolver.__init__
+ * (structured_solver.py:22-70); the block buffers themselves are owned by the
+ * caller.  use_graph != 0 captures each (operation, pointers) sequence once in
+ * a hipGraph and replays it. */
+int bta_ws_create(int device, int n, int b, int a, int use_graph, bta_ws_t* out);
+int bta_ws_destroy(bta_ws_t ws);
+/* Bytes bta_ws_create allocates for (n, b, a). */
+size_t bta_ws_bytes(int n, int b, int a);
+
+/* In-place BTA (a > 0) or BT (a == 0) Cholesky, Q = LL^T, with the
+ * log-determinant accumulated on the fly.  Replaces serinv.pobtaf(D, L, B, T)
+ * / serinv.pobtf(D, L) as called by SerinvSolver.cholesky
+ * (structured_solver.py:83-93).  On exit D[i] holds the lower factor block with
+ * zero strict upper triangle, L[i], B[i] the factor's sub-diagonal and arrow
+ * blocks, T the tip factor.  A non-positive pivot is reported through
+ * bta_ws_status (info), as LAPACK's potrf info. */
+int bta_potrf(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
+              void* stream);
+
+/* Synchronise the stream and return the status of the last bta_potrf:
+ *   info   : 0 if positive definite, else 1 + global row of the first failing
+ *            pivot (LAPACK convention);
+ *   logdet : 2 * sum log diag of the factor (the reference's
+ *            SerinvSolver.logdet, structured_solver.py:151-166; for a == 0 the
+ *            tip is excluded, as sparsity="bt" does);
+ *   err    : BTA_E_TIMEOUT if a bounded wait expired, else 0.
+ * Any output pointer may be NULL. */
+int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* stream);
+
+/* Device address of the FP64 log-determinant written by the last bta_potrf
+ * (valid after the stream reaches that point; no host synchronisation). */
+double* bta_ws_logdet_ptr(bta_ws_t ws);
+
+/* In-place triangular sweep with the factor held in D, L, B, T (as left by
+ * bta_potrf).  trans == 0 solves LL y = rhs (serinv.pobtas(..., trans="N")),
+ * trans == 1 solves LL^T x = rhs (serinv.pobtas(..., trans="C")); the pair
+ * replaces SerinvSolver.solve (structured_solver.py:102-149).  a == 0 gives
+ * pobts.  rhs is (n*b + a) x nrhs, row-major, leading dimension ldrhs
+ * (ldrhs >= nrhs, b*nrhs <= 3*b*b).  The first call after a factorization
+ * also forms W_i = D_i^-1 for every block (kept in the workspace). */
+int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
+              double* rhs, int nrhs, int ldrhs, int trans, void* stream);
+
+/* In-place selected inversion: the factor blocks are overwritten by the
+ * blocks of Sigma = Q^-1 on the BTA pattern -- D[i] <- Sigma_ii (full
+ * symmetric), L[i] <- Sigma_{i+1,i}, B[i] <- Sigma_{n,i}, T <- Sigma_T.
+ * Replaces serinv.pobtasi(D, L, B, T) / pobtsi(D, L) as called by
+ * SerinvSolver.selected_inversion (structured_solver.py:168-190). */
+int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
+              void* stream);
+
+/* Diagnostic (no reference counterpart): runs one v_mfma_f64_16x16x4f64 with
+ * A[i][k] = i + 100k, B[k][j] = j + 1000k and writes each lane's four
+ * accumulator values to out[4*lane + r] (256 doubles, device memory), so a
+ * test can confirm the operand layout every kernel here relies on. */
+int bta_selftest_mfma(double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYINLA_BTA_H */

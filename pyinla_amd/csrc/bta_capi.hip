@@ -1,0 +1,231 @@
+// extern "C" entry points of libpyinla_bta.so (declared in include/pyinla_bta.h).
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "bta_internal.h"
+#include "../../include/pyinla_bta.h"
+
+namespace {
+
+struct GraphKey {
+  int op;
+  std::vector<uintptr_t> ptrs;
+  std::vector<long long> ints;
+  bool operator<(const GraphKey& o) const { return std::tie(op, ptrs, ints) < std::tie(o.op, o.ptrs, o.ints); }
+};
+
+}  // namespace
+
+struct bta_ws {
+  bta::Workspace w;
+  int use_graph = 0;
+  hipStream_t gstream = nullptr;  // capture / replay stream
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::mutex mu;
+};
+
+namespace {
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+size_t ws_doubles(int n, int b, int a, size_t* work_out) {
+  const int nT = (b + 63) / 64;
+  // scratch for solve / selected inversion: 3 b x b blocks + arrow temporaries
+  const size_t work = 3 * (size_t)b * b + 4 * (size_t)a * b + 2 * (size_t)a * a + 64;
+  if (work_out) *work_out = work;
+  return (size_t)n * nT * 4096 + ((size_t)n * nT + 1) + 1 + 4096 + work;
+}
+
+// Run `enqueue` either directly on `st` or through a cached hipGraph.
+template <class F>
+int run_maybe_graph(bta_ws* ws, const GraphKey& key, hipStream_t st, F&& enqueue) {
+  if (!ws->use_graph) return enqueue(st);
+  std::lock_guard<std::mutex> g(ws->mu);
+  hipError_t e;
+  auto it = ws->graphs.find(key);
+  if (it == ws->graphs.end()) {
+    hipGraph_t graph;
+    e = hipStreamBeginCapture(ws->gstream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return (int)e;
+    const int rc = enqueue(ws->gstream);
+    e = hipStreamEndCapture(ws->gstream, &graph);
+    if (rc != 0) return rc;
+    if (e != hipSuccess) return (int)e;
+    hipGraphExec_t exec;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return (int)e;
+    it = ws->graphs.emplace(key, exec).first;
+  }
+  if ((e = hipEventRecord(ws->ev_in, st)) != hipSuccess) return (int)e;
+  if ((e = hipStreamWaitEvent(ws->gstream, ws->ev_in, 0)) != hipSuccess) return (int)e;
+  if ((e = hipGraphLaunch(it->second, ws->gstream)) != hipSuccess) return (int)e;
+  if ((e = hipEventRecord(ws->ev_out, ws->gstream)) != hipSuccess) return (int)e;
+  if ((e = hipStreamWaitEvent(st, ws->ev_out, 0)) != hipSuccess) return (int)e;
+  return 0;
+}
+
+int check_shape(bta_ws* ws, int n, int b, int a) {
+  if (!ws) return BTA_E_ARG;
+  // a == 0 (BT) is accepted on a BTA workspace: the arrow is then ignored.
+  if (n != ws->w.n || b != ws->w.b || (a != ws->w.a && a != 0)) return BTA_E_SHAPE;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bta_version(void) { return 100; }
+
+size_t bta_ws_bytes(int n, int b, int a) { return 8 * ws_doubles(n, b, a, nullptr) + 4 * 8 + 64; }
+
+int bta_ws_create(int device, int n, int b, int a, int use_graph, bta_ws_t* out) {
+  if (!out || n < 1 || b < 1 || a < 0 || a > 64) return BTA_E_ARG;
+  *out = nullptr;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return (int)e;
+  bta_ws* ws = new bta_ws();
+  bta::Workspace& w = ws->w;
+  w.device = device;
+  w.n = n;
+  w.b = b;
+  w.a = a;
+  w.nT = (b + 63) / 64;
+  size_t work = 0;
+  const size_t nd = ws_doubles(n, b, a, &work);
+  double* base = nullptr;
+  if ((e = hipMalloc(&base, nd * sizeof(double))) != hipSuccess) { delete ws; return (int)e; }
+  unsigned long long* u = nullptr;
+  const size_t nflags = (size_t)n * w.nT;
+  if ((e = hipMalloc(&u, (nflags + 2) * sizeof(unsigned long long) + 64)) != hipSuccess) {
+    hipFree(base);
+    delete ws;
+    return (int)e;
+  }
+  hipMemset(u, 0, (nflags + 2) * sizeof(unsigned long long) + 64);
+  hipMemset(base, 0, nd * sizeof(double));
+  w.winv = base;
+  w.logpart = w.winv + (size_t)n * w.nT * 4096;
+  w.logdet = w.logpart + (size_t)n * w.nT + 1;
+  w.tinv = w.logdet + 1;
+  w.work = w.tinv + 4096;
+  w.work_doubles = work;
+  w.flags = u;
+  w.epoch = u + nflags;
+  w.info = u + nflags + 1;
+  w.err = reinterpret_cast<int*>(u + nflags + 2);
+  ws->use_graph = use_graph;
+  if (use_graph) {
+    hipStreamCreateWithFlags(&ws->gstream, hipStreamNonBlocking);
+    hipEventCreateWithFlags(&ws->ev_in, hipEventDisableTiming);
+    hipEventCreateWithFlags(&ws->ev_out, hipEventDisableTiming);
+  }
+  if ((e = hipDeviceSynchronize()) != hipSuccess) return (int)e;
+  *out = ws;
+  return 0;
+}
+
+int bta_ws_destroy(bta_ws_t ws) {
+  if (!ws) return 0;
+  hipSetDevice(ws->w.device);
+  hipDeviceSynchronize();
+  for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);
+  if (ws->gstream) hipStreamDestroy(ws->gstream);
+  if (ws->ev_in) hipEventDestroy(ws->ev_in);
+  if (ws->ev_out) hipEventDestroy(ws->ev_out);
+  ws->w.trtri_plan.reset();
+  ws->w.potri_plan.reset();
+  hipFree(ws->w.winv);
+  hipFree(ws->w.flags);
+  if (ws->w.wfull) hipFree(ws->w.wfull);
+  delete ws;
+  return 0;
+}
+
+int bta_potrf(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a, void* stream) {
+  int rc = check_shape(ws, n, b, a);
+  if (rc) return rc;
+  if (!D || (n > 1 && !L) || (a > 0 && (!B || !T))) return BTA_E_ARG;
+  GraphKey key{1, {(uintptr_t)D, (uintptr_t)L, (uintptr_t)B, (uintptr_t)T}, {n, b, a}};
+  ws->w.w_valid = false;
+  return run_maybe_graph(ws, key, as_stream(stream), [&](hipStream_t st) {
+    return bta::potrf_enqueue(ws->w, D, L, B, T, n, b, a, st);
+  });
+}
+
+// Make the block inverses W_i of the current factor available (enqueued on st).
+static int ensure_inverses(bta_ws* ws, const double* D, int n, int b, hipStream_t st) {
+  if (ws->w.w_valid && ws->w.w_of == D) return 0;
+  int rc = bta::trtri_prepare(ws->w, D, n, b);
+  if (rc) return rc;
+  GraphKey key{3, {(uintptr_t)D}, {n, b}};
+  rc = run_maybe_graph(ws, key, st, [&](hipStream_t s2) { return bta::trtri_enqueue(ws->w, D, n, b, s2); });
+  if (rc) return rc;
+  ws->w.w_valid = true;
+  ws->w.w_of = D;
+  return 0;
+}
+
+int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a, double* rhs,
+              int nrhs, int ldrhs, int trans, void* stream) {
+  int rc = check_shape(ws, n, b, a);
+  if (rc) return rc;
+  if (!D || (n > 1 && !L) || (a > 0 && (!B || !T)) || !rhs || nrhs < 1 || ldrhs < nrhs) return BTA_E_ARG;
+  if ((size_t)b * nrhs > ws->w.work_doubles) return BTA_E_ARG;
+  if (trans != 0 && trans != 1) return BTA_E_ARG;
+  hipStream_t st = as_stream(stream);
+  rc = ensure_inverses(ws, D, n, b, st);
+  if (rc) return rc;
+  GraphKey key{2, {(uintptr_t)D, (uintptr_t)L, (uintptr_t)B, (uintptr_t)T, (uintptr_t)rhs}, {n, b, a, nrhs, ldrhs, trans}};
+  return run_maybe_graph(ws, key, st, [&](hipStream_t s2) {
+    return bta::potrs_enqueue(ws->w, D, L, B, T, n, b, a, rhs, nrhs, ldrhs, trans, s2);
+  });
+}
+
+int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a, void* stream) {
+  int rc = check_shape(ws, n, b, a);
+  if (rc) return rc;
+  if (!D || (n > 1 && !L) || (a > 0 && (!B || !T))) return BTA_E_ARG;
+  hipStream_t st = as_stream(stream);
+  rc = ensure_inverses(ws, D, n, b, st);
+  if (rc) return rc;
+  rc = bta::potri_prepare(ws->w, D, L, B, T, n, b, a);
+  if (rc) return rc;
+  GraphKey key{4, {(uintptr_t)D, (uintptr_t)L, (uintptr_t)B, (uintptr_t)T}, {n, b, a}};
+  rc = run_maybe_graph(ws, key, st, [&](hipStream_t s2) {
+    return bta::potri_enqueue(ws->w, D, L, B, T, n, b, a, s2);
+  });
+  // The factor has been overwritten by the selected inverse.
+  ws->w.w_valid = false;
+  return rc;
+}
+
+int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* stream) {
+  if (!ws) return BTA_E_ARG;
+  hipStream_t st = as_stream(stream);
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return (int)e;
+  unsigned long long inf = 0;
+  int er = 0;
+  double ld = 0.0;
+  if ((e = hipMemcpy(&inf, ws->w.info, sizeof(inf), hipMemcpyDeviceToHost)) != hipSuccess) return (int)e;
+  if ((e = hipMemcpy(&er, ws->w.err, sizeof(er), hipMemcpyDeviceToHost)) != hipSuccess) return (int)e;
+  if ((e = hipMemcpy(&ld, ws->w.logdet, sizeof(ld), hipMemcpyDeviceToHost)) != hipSuccess) return (int)e;
+  if (info) *info = (inf == bta::kInfoNone) ? 0 : (int64_t)inf + 1;
+  if (logdet) *logdet = ld;
+  if (err) *err = er ? BTA_E_TIMEOUT : 0;
+  return 0;
+}
+
+double* bta_ws_logdet_ptr(bta_ws_t ws) { return ws ? ws->w.logdet : nullptr; }
+
+int bta_selftest_mfma(double* out, void* stream) {
+  if (!out) return BTA_E_ARG;
+  return bta::selftest_mfma_enqueue(out, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,180 @@
+// Device-side building blocks for the BTA kernels (gfx950 / CDNA4, FP64).
+//
+// Everything works on 64x64 FP64 tiles held row-major in global memory with an
+// explicit leading dimension, staged through LDS, and multiplied with the
+// FP64 matrix instruction v_mfma_f64_16x16x4f64.  A workgroup is 256 threads =
+// 4 wave64s; wave w owns the 32x32 quadrant (rows 32*(w/2), cols 32*(w%2)) of a
+// 64x64 output tile as 2x2 MFMA 16x16 accumulators.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define BTA_TILE 64
+#define BTA_THREADS 256
+#define BTA_LDS_LD 66  // padded LDS row stride in doubles (bank-conflict-free fragment reads)
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Operand layout of v_mfma_f64_16x16x4f64 (D = A*B + C, A 16x4, B 4x16):
+//   A[i][k] is supplied by lane i + 16*k, B[k][j] by lane j + 16*k,
+//   D[i][j] is returned by lane j + 16*(i%4) in element i/4
+// (measured on gfx950 by bta_selftest_mfma).
+__device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+struct TileAcc {
+  f64x4 c[2][2];
+};
+
+__device__ __forceinline__ void acc_zero(TileAcc& t) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) t.c[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
+}
+
+// Row/col of accumulator element (x, y, r) for the calling thread.
+__device__ __forceinline__ int acc_row(int x, int r) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  return (w >> 1) * 32 + 16 * x + 4 * r + (lane >> 4);
+}
+__device__ __forceinline__ int acc_col(int y) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  return (w & 1) * 32 + 16 * y + (lane & 15);
+}
+
+// Load a rows x cols (<= 64 x 64) block of a row-major global matrix into the
+// 64x64 LDS tile s (stride BTA_LDS_LD), zero-filling outside the valid block.
+__device__ __forceinline__ void lds_load_tile(double* __restrict__ s, const double* __restrict__ g,
+                                              int64_t ld, int rows, int cols) {
+  const int c = threadIdx.x & 63;
+  const int r0 = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int r = r0; r < 64; r += 4) {
+    double v = 0.0;
+    if (r < rows && c < cols) v = g[(int64_t)r * ld + c];
+    s[r * BTA_LDS_LD + c] = v;
+  }
+}
+
+// Same, but store the transpose: s[c][r] = g[r][c].
+__device__ __forceinline__ void lds_load_tile_t(double* __restrict__ s, const double* __restrict__ g,
+                                                int64_t ld, int rows, int cols) {
+  const int c = threadIdx.x & 63;
+  const int r0 = threadIdx.x >> 6;
+#pragma unroll 4
+  for (int r = r0; r < 64; r += 4) {
+    double v = 0.0;
+    if (r < rows && c < cols) v = g[(int64_t)r * ld + c];
+    s[c * BTA_LDS_LD + r] = v;
+  }
+}
+
+// acc += sign * sA(64 x K) * sB(64 x K)^T over k in [0, kmax), kmax % 4 == 0.
+// sA[i][k] and sB[j][k] are LDS tiles with stride BTA_LDS_LD.
+template <bool NEG>
+__device__ __forceinline__ void mma_nt(TileAcc& t, const double* __restrict__ sA,
+                                       const double* __restrict__ sB, int kbeg, int kend) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* a0p = sA + (wm + li) * BTA_LDS_LD + lk;
+  const double* a1p = sA + (wm + 16 + li) * BTA_LDS_LD + lk;
+  const double* b0p = sB + (wn + li) * BTA_LDS_LD + lk;
+  const double* b1p = sB + (wn + 16 + li) * BTA_LDS_LD + lk;
+#pragma unroll 4
+  for (int k = kbeg; k < kend; k += 4) {
+    double a0 = a0p[k], a1 = a1p[k], b0 = b0p[k], b1 = b1p[k];
+    if (NEG) { a0 = -a0; a1 = -a1; }
+    t.c[0][0] = mfma_f64(a0, b0, t.c[0][0]);
+    t.c[0][1] = mfma_f64(a0, b1, t.c[0][1]);
+    t.c[1][0] = mfma_f64(a1, b0, t.c[1][0]);
+    t.c[1][1] = mfma_f64(a1, b1, t.c[1][1]);
+  }
+}
+
+// Load / store the accumulator from / to a row-major global block (masked).
+__device__ __forceinline__ void acc_load(TileAcc& t, const double* __restrict__ g, int64_t ld,
+                                         int rows, int cols) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = acc_row(x, r), j = acc_col(y);
+        t.c[x][y][r] = (i < rows && j < cols) ? g[(int64_t)i * ld + j] : 0.0;
+      }
+}
+
+__device__ __forceinline__ void acc_store(const TileAcc& t, double* __restrict__ g, int64_t ld,
+                                          int rows, int cols) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = acc_row(x, r), j = acc_col(y);
+        if (i < rows && j < cols) g[(int64_t)i * ld + j] = t.c[x][y][r];
+      }
+}
+
+// Store only the lower triangle (i >= j) of a diagonal tile.
+__device__ __forceinline__ void acc_store_lower(const TileAcc& t, double* __restrict__ g, int64_t ld,
+                                                int rows, int cols) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = acc_row(x, r), j = acc_col(y);
+        if (i < rows && j < cols && i >= j) g[(int64_t)i * ld + j] = t.c[x][y][r];
+      }
+}
+
+__device__ __forceinline__ void acc_to_lds(const TileAcc& t, double* __restrict__ s) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s[acc_row(x, r) * BTA_LDS_LD + acc_col(y)] = t.c[x][y][r];
+}
+
+// ------------------------------------------------------------ cross-WG flags
+//
+// Flags are 64-bit words holding the call epoch when the guarded data is ready.
+// The producer stores its data, every thread fences at agent scope (the L2s of
+// the XCDs are not coherent with each other), then one thread publishes the
+// flag with a release store.  A consumer spins (bounded) on an acquire load.
+
+__device__ __forceinline__ void flag_publish(unsigned long long* flag, unsigned long long epoch) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns false if the spin limit was hit (the caller records an error and
+// carries on so that every wave still exits).
+__device__ __forceinline__ bool flag_wait(unsigned long long* flag, unsigned long long epoch,
+                                          unsigned long long spin_limit) {
+  __shared__ int ok_sh;
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    unsigned long long it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+      if (++it > spin_limit) { ok = 0; break; }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    ok_sh = ok;
+  }
+  __syncthreads();
+  // Every thread performs its own acquire so its later loads are ordered.
+  (void)__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  const int ok = ok_sh;
+  __syncthreads();
+  return ok != 0;
+}

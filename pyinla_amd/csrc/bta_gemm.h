@@ -1,0 +1,132 @@
+// Grouped FP64 GEMM on 64x64 output tiles (FP64 MFMA), driven by a job table
+// in device memory.  One launch covers any number of independent jobs of any
+// shapes; a workgroup finds its job by binary search over the tile prefix
+// sums.  Each job computes
+//     C = beta * C + sum_s alpha_s * op(A_s) * op(B_s)
+// over up to three segments (K-concatenation), optionally only on the tiles on
+// or below the diagonal (symmetric output), optionally mirroring them so the
+// full symmetric matrix is written.  Triangular operands restrict each tile's
+// K range so structurally-zero products are skipped.
+#pragma once
+#include <vector>
+
+#include "bta_device.h"
+
+namespace bta {
+
+enum : int {
+  TRI_A_LOWER = 1,  // op(A)[i][k] == 0 for k > i   -> k < (tr+1)*64
+  TRI_A_UPPER = 2,  // op(A)[i][k] == 0 for k < i   -> k >= tr*64
+  TRI_B_LOWER = 4,  // op(B)[k][j] == 0 for k < j   -> k >= tc*64
+  TRI_B_UPPER = 8,  // op(B)[k][j] == 0 for k > j   -> k < (tc+1)*64
+};
+
+struct GemmSeg {
+  const double* A;
+  const double* B;
+  double alpha;
+  int k;       // inner dimension
+  int lda, ldb;
+  int ta, tb;  // 0: op(X) = X, 1: op(X) = X^T
+  int tri;     // TRI_* flags
+};
+
+struct GemmJob {
+  double* C;
+  double beta;   // 0: C is not read
+  int m, n, ldc;
+  int lower;     // 1: only tiles with tr >= tc
+  int mirror;    // with lower: also write the transpose into the upper triangle
+  int nseg;
+  GemmSeg seg[3];
+};
+
+// Host-side helper: builds the job table and tile prefix for one launch.
+struct GemmLaunch {
+  std::vector<GemmJob> jobs;
+  std::vector<int> prefix;  // njobs + 1
+  int tiles() const { return prefix.empty() ? 0 : prefix.back(); }
+  void add(const GemmJob& j) {
+    if (j.m <= 0 || j.n <= 0) return;
+    if (prefix.empty()) prefix.push_back(0);
+    const int tm = (j.m + 63) / 64, tn = (j.n + 63) / 64;
+    jobs.push_back(j);
+    prefix.push_back(prefix.back() + (j.lower ? tm * (tm + 1) / 2 : tm * tn));
+  }
+};
+
+inline GemmSeg seg(const double* A, int lda, int ta, const double* B, int ldb, int tb, int k,
+                   double alpha, int tri = 0) {
+  GemmSeg s;
+  s.A = A;
+  s.B = B;
+  s.alpha = alpha;
+  s.k = k;
+  s.lda = lda;
+  s.ldb = ldb;
+  s.ta = ta;
+  s.tb = tb;
+  s.tri = tri;
+  return s;
+}
+
+inline GemmJob job(double* C, int m, int n, int ldc, double beta, int lower = 0, int mirror = 0) {
+  GemmJob j;
+  j.C = C;
+  j.beta = beta;
+  j.m = m;
+  j.n = n;
+  j.ldc = ldc;
+  j.lower = lower;
+  j.mirror = mirror;
+  j.nseg = 0;
+  return j;
+}
+
+inline GemmJob& add_seg(GemmJob& j, const GemmSeg& s) {
+  if (s.k > 0) j.seg[j.nseg++] = s;
+  return j;
+}
+
+// y = beta*y + sum_s alpha_s * op(A_s) x_s, with x and y row-major k x nrhs /
+// m x nrhs blocks (leading dimensions ldx / ldy: one column per right-hand side).
+struct GemvSeg {
+  const double* A;
+  const double* x;
+  double alpha;
+  int k;      // length of x (columns of op(A))
+  int lda, ldx;
+  int ta;     // 0: A (m x k), 1: A^T (A is k x m)
+  int tri;    // 1: op(A) lower (k <= i), 2: op(A) upper (k >= i)
+};
+
+struct GemvJob {
+  double* y;
+  double beta;
+  int m, ldy, nrhs;
+  int nseg;
+  GemvSeg seg[3];
+};
+
+
+// A set of grouped-GEMM launches whose job tables live in device memory.  Built
+// once (hipMalloc + hipMemcpy, never during stream capture), then launched any
+// number of times, including from inside a captured hipGraph.
+struct DevPlan {
+  struct Entry { int job_off, pre_off, njobs, tiles; };
+  GemmJob* d_jobs = nullptr;
+  int* d_prefix = nullptr;
+  std::vector<Entry> launches;
+  DevPlan() = default;
+  DevPlan(const DevPlan&) = delete;
+  DevPlan& operator=(const DevPlan&) = delete;
+  ~DevPlan();
+  void release();
+  int build(const std::vector<GemmLaunch>& ls);
+  void launch(int idx, hipStream_t st) const;
+  bool empty() const { return launches.empty(); }
+};
+
+void gemv_launch(const GemvJob& g, int form_t, hipStream_t st);
+
+}  // namespace bta

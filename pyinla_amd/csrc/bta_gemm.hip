@@ -1,0 +1,228 @@
+// Grouped FP64 GEMM and GEMV kernels + device job-table plans.
+#include <stdexcept>
+
+#include "bta_gemm.h"
+#include "bta_internal.h"
+
+namespace bta {
+
+// acc += alpha * sA(64 x K) * sB(64 x K)^T over k in [0, kend).
+__device__ __forceinline__ void mma_nt_alpha(TileAcc& t, const double* __restrict__ sA,
+                                             const double* __restrict__ sB, int kend, double alpha) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* a0p = sA + (wm + li) * BTA_LDS_LD + lk;
+  const double* a1p = sA + (wm + 16 + li) * BTA_LDS_LD + lk;
+  const double* b0p = sB + (wn + li) * BTA_LDS_LD + lk;
+  const double* b1p = sB + (wn + 16 + li) * BTA_LDS_LD + lk;
+#pragma unroll 4
+  for (int k = 0; k < kend; k += 4) {
+    const double a0 = alpha * a0p[k], a1 = alpha * a1p[k], b0 = b0p[k], b1 = b1p[k];
+    t.c[0][0] = mfma_f64(a0, b0, t.c[0][0]);
+    t.c[0][1] = mfma_f64(a0, b1, t.c[0][1]);
+    t.c[1][0] = mfma_f64(a1, b0, t.c[1][0]);
+    t.c[1][1] = mfma_f64(a1, b1, t.c[1][1]);
+  }
+}
+
+__device__ __forceinline__ int find_job(const int* __restrict__ prefix, int njobs, int bid) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= bid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void decode_lower(int idx, int& tr, int& tc) {
+  int r = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+  while ((r + 1) * (r + 2) / 2 <= idx) ++r;
+  while (r * (r + 1) / 2 > idx) --r;
+  tr = r;
+  tc = idx - r * (r + 1) / 2;
+}
+
+__global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __restrict__ jobs,
+                                                             const int* __restrict__ prefix, int njobs) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  const int jid = find_job(prefix, njobs, blockIdx.x);
+  const GemmJob& g = jobs[jid];
+  const int local = blockIdx.x - prefix[jid];
+  int tr, tc;
+  if (g.lower) decode_lower(local, tr, tc);
+  else {
+    const int tn = (g.n + 63) / 64;
+    tr = local / tn;
+    tc = local % tn;
+  }
+  const int rows = min(64, g.m - tr * 64), cols = min(64, g.n - tc * 64);
+  double* ct = g.C + (int64_t)(tr * 64) * g.ldc + tc * 64;
+  TileAcc acc;
+  if (g.beta != 0.0) {
+    acc_load(acc, ct, g.ldc, rows, cols);
+    if (g.beta != 1.0) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc.c[x][y] *= g.beta;
+    }
+  } else {
+    acc_zero(acc);
+  }
+  for (int s = 0; s < g.nseg; ++s) {
+    const GemmSeg& sg = g.seg[s];
+    int kmin = 0, kmax = sg.k;
+    if (sg.tri & TRI_A_LOWER) kmax = min(kmax, (tr + 1) * 64);
+    if (sg.tri & TRI_A_UPPER) kmin = max(kmin, tr * 64);
+    if (sg.tri & TRI_B_LOWER) kmin = max(kmin, tc * 64);
+    if (sg.tri & TRI_B_UPPER) kmax = min(kmax, (tc + 1) * 64);
+    for (int k0 = kmin; k0 < kmax; k0 += 64) {
+      const int kw = min(64, kmax - k0);
+      if (sg.ta == 0) lds_load_tile(sA, sg.A + (int64_t)(tr * 64) * sg.lda + k0, sg.lda, rows, kw);
+      else lds_load_tile_t(sA, sg.A + (int64_t)k0 * sg.lda + tr * 64, sg.lda, kw, rows);
+      if (sg.tb == 1) lds_load_tile(sB, sg.B + (int64_t)(tc * 64) * sg.ldb + k0, sg.ldb, cols, kw);
+      else lds_load_tile_t(sB, sg.B + (int64_t)k0 * sg.ldb + tc * 64, sg.ldb, kw, cols);
+      __syncthreads();
+      mma_nt_alpha(acc, sA, sB, (kw + 3) & ~3, sg.alpha);
+      __syncthreads();
+    }
+  }
+  // epilogue
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = acc_row(x, r), j = acc_col(y);
+        if (i >= rows || j >= cols) continue;
+        const double v = acc.c[x][y][r];
+        if (g.lower && tr == tc) {
+          if (i >= j) {
+            ct[(int64_t)i * g.ldc + j] = v;
+            if (g.mirror && i > j) ct[(int64_t)j * g.ldc + i] = v;
+          }
+        } else {
+          ct[(int64_t)i * g.ldc + j] = v;
+          if (g.lower && g.mirror) g.C[(int64_t)(tc * 64 + j) * g.ldc + tr * 64 + i] = v;
+        }
+      }
+}
+
+// ------------------------------------------------------------------- GEMV
+//
+// N form (op(A) = A): one wave per output row, lanes stride the columns,
+// butterfly reduction.  T form (op(A) = A^T): 64 output columns per
+// workgroup, the 4 waves sum interleaved rows, fixed-order combine.
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// blockIdx.x covers rows in groups of 4 (N form) or columns in groups of 64 (T form).
+__global__ void __launch_bounds__(BTA_THREADS) k_gemv(GemvJob g, int form_t) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (!form_t) {
+    const int i = blockIdx.x * 4 + w;
+    if (i >= g.m) return;
+    for (int c = 0; c < g.nrhs; ++c) {
+      double acc = 0.0;
+      for (int s = 0; s < g.nseg; ++s) {
+        const GemvSeg& sg = g.seg[s];
+        int kmin = 0, kmax = sg.k;
+        if (sg.tri == 1) kmax = min(kmax, i + 1);
+        if (sg.tri == 2) kmin = i;
+        const double* a = sg.A + (int64_t)i * sg.lda;
+        double part = 0.0;
+        for (int kk = kmin + lane; kk < kmax; kk += 64) part = fma(a[kk], sg.x[(int64_t)kk * sg.ldx + c], part);
+        acc = fma(sg.alpha, wave_sum(part), acc);
+      }
+      if (lane == 0) {
+        double* yp = g.y + (int64_t)i * g.ldy + c;
+        *yp = (g.beta != 0.0) ? fma(g.beta, *yp, acc) : acc;
+      }
+    }
+  } else {
+    __shared__ double red[4][64];
+    const int j = blockIdx.x * 64 + lane;
+    for (int c = 0; c < g.nrhs; ++c) {
+      double acc = 0.0;
+      for (int s = 0; s < g.nseg; ++s) {
+        const GemvSeg& sg = g.seg[s];
+        double part = 0.0;
+        if (j < g.m) {
+          int kmin = 0, kmax = sg.k;
+          if (sg.tri == 1) kmax = min(kmax, j + 1);
+          if (sg.tri == 2) kmin = j;
+          for (int kk = kmin + w; kk < kmax; kk += 4)
+            part = fma(sg.A[(int64_t)kk * sg.lda + j], sg.x[(int64_t)kk * sg.ldx + c], part);
+        }
+        red[w][lane] = part;
+        __syncthreads();
+        if (w == 0) acc = fma(sg.alpha, ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane])), acc);
+        __syncthreads();
+      }
+      if (w == 0 && j < g.m) {
+        double* yp = g.y + (int64_t)j * g.ldy + c;
+        *yp = (g.beta != 0.0) ? fma(g.beta, *yp, acc) : acc;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------- plans
+
+DevPlan::~DevPlan() { release(); }
+
+void DevPlan::release() {
+  if (d_jobs) hipFree(d_jobs);
+  if (d_prefix) hipFree(d_prefix);
+  d_jobs = nullptr;
+  d_prefix = nullptr;
+  launches.clear();
+}
+
+int DevPlan::build(const std::vector<GemmLaunch>& ls) {
+  release();
+  std::vector<GemmJob> all;
+  std::vector<int> pre;
+  for (const auto& l : ls) {
+    Entry e;
+    e.job_off = (int)all.size();
+    e.pre_off = (int)pre.size();
+    e.njobs = (int)l.jobs.size();
+    e.tiles = l.tiles();
+    launches.push_back(e);
+    all.insert(all.end(), l.jobs.begin(), l.jobs.end());
+    pre.insert(pre.end(), l.prefix.begin(), l.prefix.end());
+  }
+  if (all.empty()) return 0;
+  hipError_t e;
+  if ((e = hipMalloc(&d_jobs, all.size() * sizeof(GemmJob))) != hipSuccess) return (int)e;
+  if ((e = hipMalloc(&d_prefix, pre.size() * sizeof(int))) != hipSuccess) return (int)e;
+  if ((e = hipMemcpy(d_jobs, all.data(), all.size() * sizeof(GemmJob), hipMemcpyHostToDevice)) != hipSuccess)
+    return (int)e;
+  if ((e = hipMemcpy(d_prefix, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return (int)e;
+  return 0;
+}
+
+void DevPlan::launch(int idx, hipStream_t st) const {
+  const Entry& e = launches[idx];
+  if (e.tiles <= 0) return;
+  hipLaunchKernelGGL(k_gemm_grouped, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
+                     d_prefix + e.pre_off, e.njobs);
+}
+
+void gemv_launch(const GemvJob& g, int form_t, hipStream_t st) {
+  if (g.m <= 0) return;
+  const int grid = form_t ? (g.m + 63) / 64 : (g.m + 3) / 4;
+  hipLaunchKernelGGL(k_gemv, dim3(grid), dim3(BTA_THREADS), 0, st, g, form_t);
+}
+
+}  // namespace bta

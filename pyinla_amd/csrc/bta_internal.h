@@ -1,0 +1,57 @@
+// Host-side internals shared by the BTA kernel files and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <memory>
+
+#include "bta_device.h"
+#include "bta_gemm.h"
+
+namespace bta {
+
+static constexpr unsigned long long kInfoNone = ~0ull;
+
+// Per-solver device workspace.  Allocated once (never inside a hot call) for a
+// fixed (n, b, a); every enqueue function only reads and writes it.
+struct Workspace {
+  int device = 0;
+  int n = 0, b = 0, a = 0, nT = 0;
+  double* winv = nullptr;             // n * nT * 64 * 64: inverse of every diagonal tile
+  double* logpart = nullptr;          // n * nT + 1 partial log sums
+  unsigned long long* flags = nullptr;  // n * nT readiness flags
+  unsigned long long* epoch = nullptr;  // call counter (device)
+  unsigned long long* info = nullptr;   // first failing pivot (global row), kInfoNone if none
+  int* err = nullptr;                   // nonzero: a bounded wait timed out
+  double* logdet = nullptr;             // last log-determinant
+  double* work = nullptr;               // scratch for solve / selected inversion
+  size_t work_doubles = 0;
+  double* tinv = nullptr;               // 64 x 64: inverse of the tip factor
+  unsigned long long spin_limit = 1ull << 24;
+  // Full inverses W_i = D_i^-1 of the diagonal factor blocks (n * b * b),
+  // allocated on first use by the solve / selected inversion.
+  double* wfull = nullptr;
+  bool w_valid = false;            // W matches the factor currently held
+  const double* w_of = nullptr;    // D pointer W was computed from
+  // cached launch plans, keyed by the block pointers they were built for
+  std::unique_ptr<DevPlan> trtri_plan, potri_plan;
+  const void* trtri_key = nullptr;
+  const void* potri_key[4] = {nullptr, nullptr, nullptr, nullptr};
+  int potri_a = -1;
+};
+
+int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
+                  hipStream_t st);
+// Host-side preparation (allocations, job tables); never called during capture.
+int trtri_prepare(Workspace& ws, const double* D, int n, int b);
+int trtri_enqueue(Workspace& ws, const double* D, int n, int b, hipStream_t st);
+int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a);
+int potri_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
+                  hipStream_t st);
+int potrs_enqueue(Workspace& ws, const double* D, const double* L, const double* B, const double* T,
+                  int n, int b, int a, double* rhs, int nrhs, int ldrhs, int trans, hipStream_t st);
+
+// Writes the 64 x 4 accumulator outputs of one v_mfma_f64_16x16x4f64 with
+// A[i][k] = i + 100 k and B[k][j] = j + 1000 k (lane-major) to out[256].
+int selftest_mfma_enqueue(double* out, hipStream_t st);
+
+}  // namespace bta

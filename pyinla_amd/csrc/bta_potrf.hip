@@ -1,0 +1,366 @@
+// BTA / BT block Cholesky (pobtaf / pobtf) and its log-determinant on gfx950.
+//
+// The factorisation of block column i is a right-looking tile Cholesky of the
+// stacked panel [D_i; L_i; B_i] (64x64 tiles), one launch per tile column k:
+//   launch k:  every trailing tile (column >= k) of the panel receives the
+//              rank-64 update from the already factored column k-1, then the
+//              tiles of column k are finished:
+//                - workgroup 0 owns the diagonal tile (k,k): update, Cholesky +
+//                  inverse in registers/LDS, store, publish a flag;
+//                - the workgroups owning the tiles below it wait on that flag
+//                  and apply X <- X * inv(L_kk)^T with FP64 MFMA.
+// Workgroups are dispatched in index order, so the flag producer (index 0) is
+// always resident before any consumer; consumers additionally spin with a
+// bounded limit and report a timeout instead of hanging.
+// After the panel, one grouped launch applies the block updates
+//   D_{i+1} -= L_i L_i^T (lower),  B_{i+1} -= B_i L_i^T,  T -= B_i B_i^T (lower)
+// and the arrowhead tip is factored at the end.  Per-tile log-pivot sums are
+// reduced in a fixed order, so the log-determinant is deterministic.
+#include "bta_internal.h"
+#include "bta_tile_chol.h"
+
+namespace bta {
+
+// ---------------------------------------------------------------- bump kernel
+
+__global__ void k_begin_call(unsigned long long* epoch, unsigned long long* info, int* err) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    epoch[0] = epoch[0] + 1ull;
+    info[0] = kInfoNone;
+    err[0] = 0;
+  }
+}
+
+// ------------------------------------------------------------ panel column k
+
+struct PanelArgs {
+  double* D;                 // D_i, b x b, ld b
+  double* L;                 // L_i (b x b) or nullptr (last block)
+  double* B;                 // B_i (a x b) or nullptr (a == 0)
+  double* Winv;              // nT tiles of 64x64: inverse of each diagonal tile
+  double* logpart;           // nT partial log sums for this block
+  unsigned long long* flags; // nT flags for this block
+  const unsigned long long* epoch;
+  unsigned long long* info;
+  int* err;
+  long long row0;            // global row of the block's first row
+  unsigned long long spin_limit;
+  int b, a, nT, nTL, nTa, k;
+};
+
+struct TileRef {
+  int kind;  // 0 = D, 1 = L, 2 = B, 3 = zero-fill (strict upper tile of D)
+  int r, j;  // tile row within its part, tile column
+};
+
+// Decode workgroup index -> tile for launch k (see file header for the order).
+__device__ __forceinline__ TileRef decode_panel_tile(const PanelArgs& p, int idx) {
+  TileRef t;
+  const int k = p.k;
+  if (idx == 0) { t.kind = 0; t.r = k; t.j = k; return t; }
+  idx -= 1;
+  // column-k tiles below the diagonal
+  const int nDb = p.nT - 1 - k;
+  if (idx < nDb) { t.kind = 0; t.r = k + 1 + idx; t.j = k; return t; }
+  idx -= nDb;
+  if (idx < p.nTL) { t.kind = 1; t.r = idx; t.j = k; return t; }
+  idx -= p.nTL;
+  if (idx < p.nTa) { t.kind = 2; t.r = idx; t.j = k; return t; }
+  idx -= p.nTa;
+  // trailing D tiles: columns j in (k, nT), rows r in [j, nT), column-major
+  for (int j = k + 1; j < p.nT; ++j) {
+    const int cnt = p.nT - j;
+    if (idx < cnt) { t.kind = 0; t.r = j + idx; t.j = j; return t; }
+    idx -= cnt;
+  }
+  const int ntr = p.nT - 1 - k;
+  if (idx < p.nTL * ntr) { t.kind = 1; t.r = idx / ntr; t.j = k + 1 + idx % ntr; return t; }
+  idx -= p.nTL * ntr;
+  if (idx < p.nTa * ntr) { t.kind = 2; t.r = idx / ntr; t.j = k + 1 + idx % ntr; return t; }
+  idx -= p.nTa * ntr;
+  // launch 0 only: zero the strict upper tiles (r < j) of D, row-major
+  t.kind = 3;
+  for (int r = 0; r < p.nT; ++r) {
+    const int cnt = p.nT - 1 - r;
+    if (idx < cnt) { t.r = r; t.j = r + 1 + idx; return t; }
+    idx -= cnt;
+  }
+  t.r = 0; t.j = 0; t.kind = 4;  // out of range (never launched)
+  return t;
+}
+
+__host__ __device__ inline int panel_grid(int nT, int nTL, int nTa, int k) {
+  const int ntr = nT - 1 - k;
+  int n = 1 + ntr + nTL + nTa + ntr * (ntr + 1) / 2 + (nTL + nTa) * ntr;
+  if (k == 0) n += nT * (nT - 1) / 2;
+  return n;
+}
+
+__global__ void __launch_bounds__(BTA_THREADS) k_panel_step(PanelArgs p) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  __shared__ double sScr[4 * 64 + 64];
+  const TileRef t = decode_panel_tile(p, blockIdx.x);
+  const unsigned long long ep = *p.epoch;
+  const int b = p.b, k = p.k;
+
+  if (t.kind == 3) {  // zero a strict-upper tile of D
+    const int rows = min(64, b - t.r * 64), cols = min(64, b - t.j * 64);
+    double* g = p.D + (int64_t)(t.r * 64) * b + t.j * 64;
+    const int c = threadIdx.x & 63;
+    for (int r = threadIdx.x >> 6; r < rows; r += 4)
+      if (c < cols) g[(int64_t)r * b + c] = 0.0;
+    return;
+  }
+  if (t.kind > 3) return;
+
+  // Pointer and extent of the target tile.
+  double* base = (t.kind == 0) ? p.D : (t.kind == 1 ? p.L : p.B);
+  const int prow = (t.kind == 2) ? p.a : b;  // rows of the part
+  const int rows = min(64, prow - t.r * 64);
+  const int cols = min(64, b - t.j * 64);
+  double* ct = base + (int64_t)(t.r * 64) * b + t.j * 64;
+
+  TileAcc acc;
+  acc_load(acc, ct, b, rows, cols);
+  if (k > 0) {
+    // rank-64 update from the factored column k-1: C -= P(r, k-1) * D(j, k-1)^T
+    const double* pa = base + (int64_t)(t.r * 64) * b + (k - 1) * 64;
+    const double* pb = p.D + (int64_t)(t.j * 64) * b + (k - 1) * 64;
+    lds_load_tile(sA, pa, b, rows, 64);
+    lds_load_tile(sB, pb, b, cols, 64);
+    __syncthreads();
+    mma_nt<true>(acc, sA, sB, 0, 64);
+    __syncthreads();
+  }
+
+  if (t.j != k) {  // trailing tile: only the update
+    acc_store(acc, ct, b, rows, cols);
+    return;
+  }
+
+  if (t.kind == 0 && t.r == k) {
+    // Diagonal tile: Cholesky + inverse, then publish.
+    acc_to_lds(acc, sA);
+    __syncthreads();
+    const int mv = rows;  // == cols
+    const TileCholResult res = tile_chol_inv(sA, sB, sScr, mv);
+    // store L_kk (full tile: zero strict upper) and inv(L_kk)
+    {
+      const int c = threadIdx.x & 63;
+      double* wk = p.Winv + (int64_t)k * 4096;
+      for (int r = threadIdx.x >> 6; r < 64; r += 4) {
+        if (r < mv && c < mv) ct[(int64_t)r * b + c] = sA[r * BTA_LDS_LD + c];
+        wk[r * 64 + c] = sB[r * BTA_LDS_LD + c];
+      }
+    }
+    if (threadIdx.x == 0) {
+      p.logpart[k] = res.logsum;
+      if (res.fail >= 0)
+        atomicMin(p.info, (unsigned long long)(p.row0 + (long long)k * 64 + res.fail));
+    }
+    flag_publish(&p.flags[k], ep);
+    return;
+  }
+
+  // Tile below the diagonal in column k: X = C * inv(L_kk)^T.
+  acc_to_lds(acc, sA);
+  if (!flag_wait(&p.flags[k], ep, p.spin_limit)) {
+    if (threadIdx.x == 0) atomicOr(p.err, 1);
+  }
+  {
+    const double* wk = p.Winv + (int64_t)k * 4096;
+    const int c = threadIdx.x & 63;
+    for (int r = threadIdx.x >> 6; r < 64; r += 4) sB[r * BTA_LDS_LD + c] = wk[r * 64 + c];
+  }
+  __syncthreads();
+  TileAcc x;
+  acc_zero(x);
+  // inv(L_kk) is lower triangular: output column c only needs k-index <= c.
+  const int kend = ((threadIdx.x >> 6) & 1) * 32 + 32;
+  mma_nt<false>(x, sA, sB, 0, kend);
+  acc_store(x, ct, b, rows, cols);
+}
+
+// ------------------------------------------------------- block update GEMMs
+
+// One grouped launch: a list of "C -= A * B^T" jobs over 64x64 output tiles,
+// each with its own pointers, extents and K.  lower != 0 restricts the job to
+// tiles on or below the diagonal (symmetric update).
+struct UpdJob {
+  double* C;
+  const double* A;
+  const double* B;
+  int m, n, kdim;     // C is m x n, A is m x kdim, B is n x kdim (all row-major)
+  int ldc, lda, ldb;
+  int lower;
+  int tile_begin;     // first flat tile index of this job
+  int tiles_n;        // tiles along n
+};
+
+#define BTA_MAX_JOBS 8
+struct UpdJobs {
+  UpdJob job[BTA_MAX_JOBS];
+  int njobs;
+};
+
+__device__ __forceinline__ void decode_job_tile(const UpdJob& g, int idx, int& tr, int& tc) {
+  if (g.lower) {
+    // row-major over the lower triangle: row r has r+1 tiles
+    int r = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= idx) ++r;
+    while (r * (r + 1) / 2 > idx) --r;
+    tr = r;
+    tc = idx - r * (r + 1) / 2;
+  } else {
+    tr = idx / g.tiles_n;
+    tc = idx % g.tiles_n;
+  }
+}
+
+__global__ void __launch_bounds__(BTA_THREADS) k_gemm_nt_sub(UpdJobs jobs) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  int jid = 0;
+  for (int q = 1; q < jobs.njobs; ++q)
+    if ((int)blockIdx.x >= jobs.job[q].tile_begin) jid = q;
+  const UpdJob g = jobs.job[jid];
+  int tr, tc;
+  decode_job_tile(g, blockIdx.x - g.tile_begin, tr, tc);
+  const int rows = min(64, g.m - tr * 64), cols = min(64, g.n - tc * 64);
+  if (rows <= 0 || cols <= 0) return;
+  double* ct = g.C + (int64_t)(tr * 64) * g.ldc + tc * 64;
+  const double* at = g.A + (int64_t)(tr * 64) * g.lda;
+  const double* bt = g.B + (int64_t)(tc * 64) * g.ldb;
+  TileAcc acc;
+  acc_load(acc, ct, g.ldc, rows, cols);
+  for (int k0 = 0; k0 < g.kdim; k0 += 64) {
+    const int kw = min(64, g.kdim - k0);
+    lds_load_tile(sA, at + k0, g.lda, rows, kw);
+    lds_load_tile(sB, bt + k0, g.ldb, cols, kw);
+    __syncthreads();
+    mma_nt<true>(acc, sA, sB, 0, (kw + 3) & ~3);
+    __syncthreads();
+  }
+  if (g.lower && tr == tc) acc_store_lower(acc, ct, g.ldc, rows, cols);
+  else acc_store(acc, ct, g.ldc, rows, cols);
+}
+
+// ------------------------------------------------------------- tip / logdet
+
+__global__ void __launch_bounds__(BTA_THREADS) k_tip_potrf(double* T, int a, double* logpart_tip,
+                                                          unsigned long long* info, long long row0,
+                                                          double* tinv) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  __shared__ double sScr[4 * 64 + 64];
+  lds_load_tile(sA, T, a, a, a);
+  __syncthreads();
+  const TileCholResult res = tile_chol_inv(sA, sB, sScr, a);
+  const int c = threadIdx.x & 63;
+  for (int r = threadIdx.x >> 6; r < 64; r += 4) {
+    if (r < a && c < a) T[(int64_t)r * a + c] = sA[r * BTA_LDS_LD + c];
+    tinv[r * 64 + c] = sB[r * BTA_LDS_LD + c];
+  }
+  if (threadIdx.x == 0) {
+    logpart_tip[0] = res.logsum;
+    if (res.fail >= 0) atomicMin(info, (unsigned long long)(row0 + res.fail));
+  }
+}
+
+// Ordered reduction of npart partial sums: thread t sums t, t+256, ... in
+// order, then a fixed pairwise tree.  out = 2 * sum.
+__global__ void __launch_bounds__(BTA_THREADS) k_logdet_reduce(const double* part, int npart, double* out) {
+  __shared__ double s[BTA_THREADS];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < npart; i += BTA_THREADS) acc += part[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = BTA_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = 2.0 * s[0];
+}
+
+// ------------------------------------------------------------ layout check
+
+__global__ void k_selftest_mfma(double* out) {
+  const int l = threadIdx.x;
+  const double a = (double)(l % 16) + 100.0 * (l / 16);   // A[i][k], i = l%16, k = l/16
+  const double b = (double)(l % 16) + 1000.0 * (l / 16);  // B[k][j], j = l%16, k = l/16
+  f64x4 c = {0.0, 0.0, 0.0, 0.0};
+  c = mfma_f64(a, b, c);
+  for (int r = 0; r < 4; ++r) out[l * 4 + r] = c[r];
+}
+
+int selftest_mfma_enqueue(double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, st, out);
+  return (int)hipGetLastError();
+}
+
+// -------------------------------------------------------------- host driver
+
+static void launch_gemm_jobs(UpdJobs& jobs, hipStream_t st) {
+  int total = 0;
+  for (int q = 0; q < jobs.njobs; ++q) {
+    UpdJob& g = jobs.job[q];
+    const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
+    g.tiles_n = tn;
+    g.tile_begin = total;
+    total += g.lower ? tm * (tm + 1) / 2 : tm * tn;
+  }
+  if (total > 0) hipLaunchKernelGGL(k_gemm_nt_sub, dim3(total), dim3(BTA_THREADS), 0, st, jobs);
+}
+
+int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
+                  hipStream_t st) {
+  const int nT = (b + 63) / 64;
+  const int nTa = (a + 63) / 64;
+  hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, st, ws.epoch, ws.info, ws.err);
+  for (int i = 0; i < n; ++i) {
+    PanelArgs p;
+    p.D = D + (int64_t)i * b * b;
+    p.L = (i < n - 1) ? L + (int64_t)i * b * b : nullptr;
+    p.B = (a > 0) ? B + (int64_t)i * a * b : nullptr;
+    p.Winv = ws.winv + (int64_t)i * nT * 4096;
+    p.logpart = ws.logpart + (int64_t)i * nT;
+    p.flags = ws.flags + (int64_t)i * nT;
+    p.epoch = ws.epoch;
+    p.info = ws.info;
+    p.err = ws.err;
+    p.row0 = (long long)i * b;
+    p.spin_limit = ws.spin_limit;
+    p.b = b;
+    p.a = a;
+    p.nT = nT;
+    p.nTL = p.L ? nT : 0;
+    p.nTa = p.B ? nTa : 0;
+    for (int k = 0; k < nT; ++k) {
+      p.k = k;
+      hipLaunchKernelGGL(k_panel_step, dim3(panel_grid(nT, p.nTL, p.nTa, k)), dim3(BTA_THREADS), 0, st, p);
+    }
+    UpdJobs jobs;
+    jobs.njobs = 0;
+    if (i < n - 1) {
+      double* Dn = D + (int64_t)(i + 1) * b * b;
+      jobs.job[jobs.njobs++] = UpdJob{Dn, p.L, p.L, b, b, b, b, b, b, 1, 0, 0};
+      if (a > 0) {
+        double* Bn = B + (int64_t)(i + 1) * a * b;
+        jobs.job[jobs.njobs++] = UpdJob{Bn, p.B, p.L, a, b, b, b, b, b, 0, 0, 0};
+      }
+    }
+    if (a > 0) jobs.job[jobs.njobs++] = UpdJob{T, p.B, p.B, a, a, b, a, b, b, 1, 0, 0};
+    launch_gemm_jobs(jobs, st);
+  }
+  int npart = n * nT;
+  if (a > 0) {
+    hipLaunchKernelGGL(k_tip_potrf, dim3(1), dim3(BTA_THREADS), 0, st, T, a, ws.logpart + npart, ws.info,
+                       (long long)n * b, ws.tinv);
+    npart += 1;
+  }
+  hipLaunchKernelGGL(k_logdet_reduce, dim3(1), dim3(BTA_THREADS), 0, st, ws.logpart, npart, ws.logdet);
+  return (int)hipGetLastError();
+}
+
+}  // namespace bta

@@ -1,0 +1,171 @@
+// Block inverses of the diagonal factor blocks and the BTA selected inversion
+// (pobtasi / pobtsi) on gfx950.
+//
+// W_i = D_i^-1 for every diagonal factor block is built bottom-up from the
+// 64x64 tile inverses the factorization already produced: ranges of 2^s tiles
+// are merged pairwise with
+//     [[W11, 0], [W21, W22]],  W21 = -W22 (L21 W11),
+// where T1^T = W11^T L21^T is parked in the (still unused) upper triangle of
+// W_i.  Every level is two grouped-GEMM launches over all pairs of all n
+// blocks, so the whole inverse costs ~2 log2(b/64) launches.
+//
+// Selected inversion then runs the Takahashi recurrence backwards in i with
+// G = L_i W_i and Ga = B_i W_i (three grouped launches per block):
+//     Sigma_{n,i}   = -Sigma_{n,i+1} G - Sigma_T Ga
+//     Sigma_{i+1,i} = -Sigma_{i+1,i+1} G - Sigma_{n,i+1}^T Ga
+//     Sigma_ii      =  W_i^T W_i - G^T Sigma_{i+1,i} - Ga^T Sigma_{n,i}
+// with Sigma_T = T^-T T^-1.  Sigma_ii is written full symmetric.
+#include "bta_internal.h"
+
+namespace bta {
+
+__global__ void __launch_bounds__(BTA_THREADS) k_leaf_copy(double* W, const double* winv, int b, int nT) {
+  const int blk = blockIdx.x / nT, k = blockIdx.x % nT;
+  const int mv = min(64, b - k * 64);
+  const double* src = winv + ((int64_t)blk * nT + k) * 4096;
+  double* dst = W + (int64_t)blk * b * b + (int64_t)(k * 64) * b + k * 64;
+  const int c = threadIdx.x & 63;
+  for (int r = threadIdx.x >> 6; r < mv; r += 4)
+    if (c < mv) dst[(int64_t)r * b + c] = src[r * 64 + c];
+}
+
+// Zero the strict upper triangle of n b x b blocks (one workgroup per 64-row strip).
+__global__ void __launch_bounds__(BTA_THREADS) k_zero_upper(double* W, int b, int nT) {
+  const int blk = blockIdx.x / nT, rt = blockIdx.x % nT;
+  double* base = W + (int64_t)blk * b * b;
+  const int rows = min(64, b - rt * 64);
+  for (int r = 0; r < rows; ++r) {
+    const int i = rt * 64 + r;
+    double* row = base + (int64_t)i * b;
+    for (int j = i + 1 + threadIdx.x; j < b; j += BTA_THREADS) row[j] = 0.0;
+  }
+}
+
+int trtri_prepare(Workspace& ws, const double* D, int n, int b) {
+  if (!ws.wfull) {
+    hipError_t e = hipMalloc(&ws.wfull, (size_t)n * b * b * sizeof(double));
+    if (e != hipSuccess) return (int)e;
+  }
+  if (ws.trtri_plan && ws.trtri_key == D) return 0;
+  const int nT = (b + 63) / 64;
+  std::vector<GemmLaunch> ls;
+  for (int s = 1; s < nT; s *= 2) {
+    GemmLaunch la, lb;
+    for (int t0 = 0; t0 + s < nT; t0 += 2 * s) {
+      const int r1 = t0 * 64, r2 = (t0 + s) * 64;
+      const int m1 = s * 64;
+      const int m2 = min((t0 + 2 * s) * 64, b) - r2;
+      for (int i = 0; i < n; ++i) {
+        double* W = ws.wfull + (int64_t)i * b * b;
+        const double* Di = D + (int64_t)i * b * b;
+        // T1^T = W11^T L21^T  -> upper block (r1, r2), m1 x m2
+        GemmJob ja = job(W + (int64_t)r1 * b + r2, m1, m2, b, 0.0);
+        add_seg(ja, seg(W + (int64_t)r1 * b + r1, b, 1, Di + (int64_t)r2 * b + r1, b, 1, m1, 1.0, TRI_A_UPPER));
+        la.add(ja);
+        // W21 = -W22 T1  -> lower block (r2, r1), m2 x m1
+        GemmJob jb = job(W + (int64_t)r2 * b + r1, m2, m1, b, 0.0);
+        add_seg(jb, seg(W + (int64_t)r2 * b + r2, b, 0, W + (int64_t)r1 * b + r2, b, 1, m2, -1.0, TRI_A_LOWER));
+        lb.add(jb);
+      }
+    }
+    ls.push_back(std::move(la));
+    ls.push_back(std::move(lb));
+  }
+  ws.trtri_plan.reset(new DevPlan());
+  const int rc = ws.trtri_plan->build(ls);
+  if (rc) {
+    ws.trtri_plan.reset();
+    return rc;
+  }
+  ws.trtri_key = D;
+  return 0;
+}
+
+int trtri_enqueue(Workspace& ws, const double* D, int n, int b, hipStream_t st) {
+  const int nT = (b + 63) / 64;
+  hipLaunchKernelGGL(k_leaf_copy, dim3(n * nT), dim3(BTA_THREADS), 0, st, ws.wfull, ws.winv, b, nT);
+  for (size_t q = 0; q < ws.trtri_plan->launches.size(); ++q) ws.trtri_plan->launch((int)q, st);
+  hipLaunchKernelGGL(k_zero_upper, dim3(n * nT), dim3(BTA_THREADS), 0, st, ws.wfull, b, nT);
+  return (int)hipGetLastError();
+}
+
+int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a) {
+  int rc = trtri_prepare(ws, D, n, b);
+  if (rc) return rc;
+  const void* key[4] = {D, L, B, T};
+  bool same = ws.potri_plan && ws.potri_a == a;
+  for (int q = 0; q < 4 && same; ++q) same = (ws.potri_key[q] == key[q]);
+  if (same) return 0;
+  double* G = ws.work;                      // b x b
+  double* Ga = ws.work + (size_t)b * b;     // a x b
+  std::vector<GemmLaunch> ls;
+  if (a > 0) {  // Sigma_T = Tinv^T Tinv
+    GemmLaunch l0;
+    GemmJob j = job(T, a, a, a, 0.0);
+    add_seg(j, seg(ws.tinv, 64, 1, ws.tinv, 64, 0, a, 1.0));
+    l0.add(j);
+    ls.push_back(std::move(l0));
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    const bool hasL = i < n - 1;
+    double* Di = D + (int64_t)i * b * b;
+    double* Wi = ws.wfull + (int64_t)i * b * b;
+    double* Li = hasL ? L + (int64_t)i * b * b : nullptr;
+    double* Bi = a > 0 ? B + (int64_t)i * a * b : nullptr;
+    double* Dn = hasL ? D + (int64_t)(i + 1) * b * b : nullptr;
+    double* Bn = (hasL && a > 0) ? B + (int64_t)(i + 1) * a * b : nullptr;
+    GemmLaunch l1, l2, l3;
+    if (hasL) {
+      GemmJob j = job(G, b, b, b, 0.0);
+      add_seg(j, seg(Li, b, 0, Wi, b, 0, b, 1.0, TRI_B_LOWER));
+      l1.add(j);
+    }
+    if (a > 0) {
+      GemmJob j = job(Ga, a, b, b, 0.0);
+      add_seg(j, seg(Bi, b, 0, Wi, b, 0, b, 1.0, TRI_B_LOWER));
+      l1.add(j);
+      GemmJob s = job(Bi, a, b, b, 0.0);  // Sigma_{n,i}
+      if (hasL) add_seg(s, seg(Bn, b, 0, G, b, 0, b, -1.0));
+      add_seg(s, seg(T, a, 0, Ga, b, 0, a, -1.0));
+      l2.add(s);
+    }
+    if (hasL) {
+      GemmJob s = job(Li, b, b, b, 0.0);  // Sigma_{i+1,i}
+      add_seg(s, seg(Dn, b, 0, G, b, 0, b, -1.0));
+      if (a > 0) add_seg(s, seg(Bn, b, 1, Ga, b, 0, a, -1.0));
+      l2.add(s);
+    }
+    GemmJob sd = job(Di, b, b, b, 0.0, /*lower=*/1, /*mirror=*/1);  // Sigma_ii
+    add_seg(sd, seg(Wi, b, 1, Wi, b, 0, b, 1.0, TRI_A_UPPER | TRI_B_LOWER));
+    if (hasL) add_seg(sd, seg(G, b, 1, Li, b, 0, b, -1.0));
+    if (a > 0) add_seg(sd, seg(Ga, b, 1, Bi, b, 0, a, -1.0));
+    l3.add(sd);
+    ls.push_back(std::move(l1));
+    ls.push_back(std::move(l2));
+    ls.push_back(std::move(l3));
+  }
+  ws.potri_plan.reset(new DevPlan());
+  rc = ws.potri_plan->build(ls);
+  if (rc) {
+    ws.potri_plan.reset();
+    return rc;
+  }
+  for (int q = 0; q < 4; ++q) ws.potri_key[q] = key[q];
+  ws.potri_a = a;
+  return 0;
+}
+
+int potri_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
+                  hipStream_t st) {
+  (void)L;
+  (void)B;
+  (void)T;
+  if (!ws.w_valid || ws.w_of != D) {
+    const int rc = trtri_enqueue(ws, D, n, b, st);
+    if (rc) return rc;
+  }
+  for (size_t q = 0; q < ws.potri_plan->launches.size(); ++q) ws.potri_plan->launch((int)q, st);
+  return (int)hipGetLastError();
+}
+
+}  // namespace bta

@@ -1,0 +1,135 @@
+"""Seeded synthetic inputs for the BTA solver path.
+
+Every generator uses ``numpy.random.default_rng(seed)`` with the draw order
+written in its docstring; "N(mu, s)" means mean mu and standard deviation s.
+These are the inputs the parity tests and the benchmark run on; the reference
+ships no usable data (its example inputs are git-LFS pointers).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+# ----------------------------------------------------------- known-factor BTA
+
+@dataclass
+class KnownFactorBTA:
+    """A BTA matrix Q = LL^T built from a known block factor.
+
+    ``Lf_*`` are the factor blocks (diag lower-triangular, sub-diagonal, arrow,
+    tip), ``Q_*`` the blocks of Q in the reference storage layout, ``rhs`` one
+    right-hand side of length N = n*b + a.
+    """
+    n: int
+    b: int
+    a: int
+    Lf_D: np.ndarray
+    Lf_L: np.ndarray
+    Lf_B: np.ndarray
+    Lf_T: np.ndarray
+    Q_D: np.ndarray
+    Q_L: np.ndarray
+    Q_B: np.ndarray
+    Q_T: np.ndarray
+    rhs: np.ndarray
+
+
+def known_factor_draws(rng: np.random.Generator, n: int, b: int, a: int):
+    """Draw the factor blocks in a fixed order.
+
+    Draw order: (1) diag-block strict-lower entries N(0, 0.1), shape (n, b, b);
+    (2) diag-block diagonals |N(0,1)| + 1, shape (n, b); (3) sub-diagonal blocks
+    N(0, 0.1), shape (n-1, b, b); (4) arrow blocks N(0, 0.05), shape (n, a, b);
+    (5) tip strict-lower N(0, 0.05), shape (a, a); (6) tip diagonal
+    1 + |N(0,1)|, shape (a,); (7) rhs N(0,1), shape (n*b + a,).
+    """
+    Ld = np.tril(rng.normal(0.0, 0.1, (n, b, b)), -1)
+    dg = np.abs(rng.normal(0.0, 1.0, (n, b))) + 1.0
+    idx = np.arange(b)
+    Ld[:, idx, idx] = dg
+    Ll = rng.normal(0.0, 0.1, (max(n - 1, 0), b, b))
+    Lb = rng.normal(0.0, 0.05, (n, a, b))
+    Lt = np.tril(rng.normal(0.0, 0.05, (a, a)), -1)
+    Lt[np.arange(a), np.arange(a)] = 1.0 + np.abs(rng.normal(0.0, 1.0, a))
+    rhs = rng.normal(0.0, 1.0, n * b + a)
+    return Ld, Ll, Lb, Lt, rhs
+
+
+def q_from_factor(Ld, Ll, Lb, Lt):
+    """Blocks of Q = LL^T for a block lower-bidiagonal-arrow factor.
+
+    Q[i,i]   = Ld_i Ld_i^T + Ll_{i-1} Ll_{i-1}^T
+    Q[i+1,i] = Ll_i Ld_i^T
+    Q[n,i]   = Lb_i Ld_i^T + Lb_{i-1} Ll_{i-1}^T
+    Q[n,n]   = Lt Lt^T + sum_i Lb_i Lb_i^T
+    Works on numpy arrays or torch tensors (uses only @, .transpose and copies).
+    """
+    n = Ld.shape[0]
+    tr = (lambda X: np.swapaxes(X, -1, -2) if isinstance(X, np.ndarray) else X.transpose(-1, -2))
+    QD = Ld @ tr(Ld)
+    if n > 1:
+        QD[1:] += Ll @ tr(Ll)
+        QL = Ll @ tr(Ld[:-1])
+    else:
+        QL = Ll[:0] * 0
+    QB = Lb @ tr(Ld)
+    if n > 1:
+        QB[1:] += Lb[:-1] @ tr(Ll)
+    QT = Lt @ tr(Lt) + (Lb @ tr(Lb)).sum(0)
+    return QD, QL, QB, QT
+
+
+def known_factor_bta(n: int = 50, b: int = 92, a: int = 4, seed: int = 0) -> KnownFactorBTA:
+    """Plain BTA test matrix: n diagonal blocks of size b, arrowhead a.
+
+    Defaults are the plain FP64 BTA solve case (n=50, b=92, a=4, seed 0).
+    """
+    rng = np.random.default_rng(seed)
+    Ld, Ll, Lb, Lt, rhs = known_factor_draws(rng, n, b, a)
+    QD, QL, QB, QT = q_from_factor(Ld, Ll, Lb, Lt)
+    return KnownFactorBTA(n, b, a, Ld, Ll, Lb, Lt, QD, QL, QB, QT, rhs)
+
+
+def known_factor_bta_torch(n: int, b: int, a: int, seed: int, device):
+    """Large known-factor BTA matrix generated on ``device`` with torch.
+
+    Used where numpy generation would take minutes (b in the thousands).  Same
+    distributions as :func:`known_factor_draws` but drawn from a torch
+    generator seeded with ``seed`` on ``device``; returns (factor, Q) tuples of
+    float64 tensors (D, L, B, T).
+    """
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    f64 = torch.float64
+    Ld = torch.randn((n, b, b), generator=g, device=device, dtype=f64).mul_(0.1).tril_(-1)
+    dg = torch.randn((n, b), generator=g, device=device, dtype=f64).abs_().add_(1.0)
+    Ld.diagonal(dim1=1, dim2=2).copy_(dg)
+    Ll = torch.randn((max(n - 1, 0), b, b), generator=g, device=device, dtype=f64).mul_(0.1)
+    Lb = torch.randn((n, a, b), generator=g, device=device, dtype=f64).mul_(0.05)
+    Lt = torch.randn((a, a), generator=g, device=device, dtype=f64).mul_(0.05).tril_(-1)
+    Lt.diagonal().copy_(torch.randn((a,), generator=g, device=device, dtype=f64).abs_().add_(1.0))
+    Q = _q_from_factor_torch(Ld, Ll, Lb, Lt)
+    return (Ld, Ll, Lb, Lt), Q
+
+
+def _q_from_factor_torch(Ld, Ll, Lb, Lt):
+    """Blockwise Q = LL^T on torch tensors, one block at a time (bounded memory)."""
+    import torch
+    n, b = Ld.shape[0], Ld.shape[1]
+    QD = torch.empty_like(Ld)
+    QL = torch.empty_like(Ll)
+    QB = torch.empty_like(Lb)
+    QT = Lt @ Lt.T
+    for i in range(n):
+        QD[i] = Ld[i] @ Ld[i].T.contiguous()
+        QB[i] = Lb[i] @ Ld[i].T.contiguous()
+        if i > 0:
+            QD[i] += Ll[i - 1] @ Ll[i - 1].T.contiguous()
+            QB[i] += Lb[i - 1] @ Ll[i - 1].T.contiguous()
+        if i < n - 1:
+            QL[i] = Ll[i] @ Ld[i].T.contiguous()
+        QT += Lb[i] @ Lb[i].T.contiguous()
+    return QD, QL, QB, QT
