@@ -1,0 +1,231 @@
+#!/usr/bin/env python
+"""Benchmark of the BTA solver path on one MI355X.
+
+Workload (default): the Gaussian spatio-temporal model with nt = 200 time
+steps, ns = 2000 mesh nodes (50 x 40 grid, hx = 1/49, hy = 1/39), nb = 6 fixed
+effects, dim(theta) = 4, m = 4 ns nt observations.  One step = one f(theta) +
+grad f evaluation.  ``value`` is adjoint evaluations per second (forward:
+assemble Q_c and Q_p, two FP64 BTA Cholesky factorizations, log-determinants,
+one solve; backward: two selected inversions in place plus trace
+contractions).  The finite-difference path (2 dim(theta) + 1 = 9 f
+evaluations, 18 factorizations, as the reference's _objective_function) is
+timed beside it.  All inputs are resident on the GPU before the timed region.
+
+Prints one JSON line (rank 0, N = 1 only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "f(θ)+∇f evals/s, adjoint vs FD; pobtaf/pobtasi FP64 TFLOP/s vs peak"
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def flops_potrf(n, b, a):
+    """Algorithmic FLOPs of the BTA Cholesky (a = 0: BT variant)."""
+    return (n * b**3 / 3 + (n - 1) * 2 * b**3 + n * (a * b**2 + a**2 * b)
+            + (n - 1) * 2 * a * b**2 + a**3 / 3)
+
+
+def flops_potrf_split(n, b, a):
+    """(panel, update) parts of flops_potrf: the per-column panel kernel does
+    the diagonal-block Cholesky and both TRSMs, the update kernel the SYRK /
+    GEMM updates of the next block, arrow and tip."""
+    panel = n * b**3 / 3 + (n - 1) * b**3 + n * a * b**2
+    update = (n - 1) * b**3 + n * a**2 * b + (n - 1) * 2 * a * b**2
+    return panel, update
+
+
+def flops_potri(n, b, a):
+    """Algorithmic FLOPs of the BTA selected inversion (lower-only Sigma_ii update)."""
+    return n * (2 / 3) * b**3 + (n - 1) * 4 * b**3 + n * (7 * a * b**2 + 2 * a**2 * b) + a**3
+
+
+def bytes_potrs(n, b, a):
+    return 16 * (n * b**2 + (n - 1) * b**2 + n * a * b + a**2)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt", type=int, default=200)
+    ap.add_argument("--nx", type=int, default=50)
+    ap.add_argument("--ny", type=int, default=40)
+    ap.add_argument("--nb", type=int, default=6)
+    ap.add_argument("--fd-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-nt", type=int, default=4, help="time steps of the bounded CPU sample")
+    ap.add_argument("--no-graph", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(args, prob_full):
+    """Oracle (numpy/scipy BTA restatement + the reference's f and FD loop) on
+    a bounded sample: the same mesh with cpu_nt time steps; the FD gradient
+    cost is extrapolated linearly in nt to the full workload."""
+    from oracle import model as om
+    from pyinla_amd.coefficients import CoefficientModel
+    from pyinla_amd.workloads import st_problem
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    nts = args.cpu_nt
+    prob = st_problem(nt=nts, nx=args.nx, ny=args.ny, nb=args.nb, seed=3,
+                      hx=1.0 / (args.nx - 1), hy=1.0 / (args.ny - 1))
+    cm = CoefficientModel("baseline")
+    om.evaluate_f(prob, cm.table, prob.theta)  # warm-up
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        om.evaluate_f(prob, cm.table, prob.theta)
+        times.append(time.perf_counter() - t0)
+    t_f = float(np.median(times))
+    d = 4
+    t_grad_full = (2 * d + 1) * t_f * (args.nt / nts)
+    return {"value": 1.0 / t_grad_full, "unit": "evals/s", "cores": int(threads), "kind": "port",
+            "sample": (f"oracle f(theta) (numpy/scipy BTA restatement, LAPACK/BLAS with {threads} threads) "
+                       f"at nt={nts} on the same {args.nx}x{args.ny} mesh, median of 3 = {t_f:.3f} s; "
+                       f"FD gradient = {2 * d + 1} f evaluations (reference order), extrapolated linearly "
+                       f"in nt to nt={args.nt}"),
+            "sample_f_seconds": t_f, "extrapolated_fd_grad_seconds": t_grad_full}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no "
+              "multi-GPU path. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from pyinla_amd import _lib
+    from pyinla_amd.st_model import STGaussianModel
+    from pyinla_amd.workloads import st_problem
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    lib = _lib.load()
+    t0 = time.perf_counter()
+    prob = st_problem(nt=args.nt, nx=args.nx, ny=args.ny, nb=args.nb, seed=3,
+                      hx=1.0 / (args.nx - 1), hy=1.0 / (args.ny - 1))
+    n, b, a = prob.nt, prob.ns, prob.nb
+    log(f"problem nt={n} ns={b} nb={a} m={prob.A.shape[0]} built in {time.perf_counter() - t0:.1f}s")
+    model = STGaussianModel(prob, device=dev, use_graph=not args.no_graph)
+    theta = prob.theta.copy()
+    log("model resident on GPU")
+
+    for w in range(args.warmup):
+        f0, g_adj = model.value_and_grad(theta)
+        torch.cuda.synchronize()
+        log(f"warmup {w}: f={f0:.10e}")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f_adj, g_adj = model.value_and_grad(theta)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / args.steps
+    log(f"adjoint f+grad: {t_adj * 1e3:.1f} ms/step")
+
+    # plain GPU FD path (9 f evaluations, reference order), step 1e-3
+    f_fd, g_fd = model.fd_gradient(theta, eps=1e-3)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fd_steps):
+        f_fd, g_fd = model.fd_gradient(theta, eps=1e-3)
+    torch.cuda.synchronize()
+    t_fd = (time.perf_counter() - t0) / args.fd_steps
+    log(f"FD f+grad: {t_fd * 1e3:.1f} ms/step")
+    grad_rel = float(np.abs(g_adj - g_fd).max() / max(np.abs(g_fd).max(), 1e-300))
+
+    # live per-kernel-class timing (HIP events on the launch stream) of one
+    # adjoint evaluation
+    classes = {0: "panel", 1: "update", 2: "grouped", 3: "gemv"}
+    tot = {c: 0.0 for c in classes}
+    cnt = {c: 0 for c in classes}
+    import ctypes as C
+    for s in (model.cond, model.prior):
+        _lib.check(lib.bta_ws_set_timing(s._ws, 1), "set_timing")
+    model.value_and_grad(theta)
+    torch.cuda.synchronize()
+    for s in (model.cond, model.prior):
+        for c in classes:
+            ms = C.c_double()
+            k = C.c_int64()
+            _lib.check(lib.bta_ws_get_timing(s._ws, c, C.byref(ms), C.byref(k)), "get_timing")
+            tot[c] += ms.value
+            cnt[c] += k.value
+        _lib.check(lib.bta_ws_set_timing(s._ws, 0), "set_timing")
+    fp_c, fu_c = flops_potrf_split(n, b, a)
+    fp_p, fu_p = flops_potrf_split(n, b, 0)
+    flops_cls = {0: fp_c + fp_p, 1: fu_c + fu_p, 2: flops_potri(n, b, a) + flops_potri(n, b, 0)}
+    potrf_ms = tot[0] + tot[1]
+    potrf_tflops = (flops_potrf(n, b, a) + flops_potrf(n, b, 0)) / (potrf_ms * 1e-3) / 1e12 if potrf_ms else None
+    potri_tflops = flops_cls[2] / (tot[2] * 1e-3) / 1e12 if tot[2] else None
+    dom = max(classes, key=lambda c: tot[c])
+    if dom == 3:
+        ach = bytes_potrs(n, b, a) / (tot[3] * 1e-3 / max(cnt[3], 1) * cnt[3]) / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS,
+                "traffic": None}
+    else:
+        per_launch_flops = flops_cls[dom] / max(cnt[dom], 1)
+        avg_ms = tot[dom] / max(cnt[dom], 1)
+        ach = per_launch_flops / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
+    roof["kernel"] = {0: "k_panel_step", 1: "k_gemm_nt_sub", 2: "k_gemm_grouped", 3: "k_gemv"}[dom]
+    roof["launches"] = cnt[dom]
+    roof["avg_launch_ms"] = tot[dom] / max(cnt[dom], 1)
+    breakdown = {classes[c]: {"ms": tot[c], "launches": cnt[c]} for c in classes}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (bounded sample)")
+        cpu = cpu_baseline(args, prob)
+
+    out = {
+        "metric": METRIC,
+        "value": 1.0 / t_adj,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_adj * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"gaussian spatio-temporal f+grad (adjoint), nt={n}, ns={b} "
+                               f"({args.nx}x{args.ny} grid), nb={a}, dim_theta=4",
+                   "nt": n, "ns": b, "nb": a, "dim_theta": 4, "observations": int(prob.A.shape[0])},
+        "fd": {"value": 1.0 / t_fd, "unit": "evals/s", "ms_per_step": t_fd * 1e3,
+               "adjoint_speedup": t_fd / t_adj, "grad_rel_diff_adjoint_vs_fd1e-3": grad_rel},
+        "pobtaf_tflops": potrf_tflops,
+        "pobtaf_frac_peak": potrf_tflops / PEAK_FP64_TFLOPS if potrf_tflops else None,
+        "pobtasi_tflops": potri_tflops,
+        "pobtasi_frac_peak": potri_tflops / PEAK_FP64_TFLOPS if potri_tflops else None,
+        "kernel_ms": breakdown,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "f": f_adj,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,17 @@ int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* st
  * (valid after the stream reaches that point; no host synchronisation). */
 double* bta_ws_logdet_ptr(bta_ws_t ws);
 
+/* Optional event timing (no reference counterpart; the reference times whole
+ * calls with perf_counter, structured_solver.py:80-100).  While on, graphs are
+ * bypassed and every launch of the classes 0 = panel factorization step,
+ * 1 = block update GEMM, 2 = grouped GEMM (inverses / selected inversion),
+ * 3 = GEMV sweep, 4 = other is bracketed by a pair of hipEvents on its stream.
+ * bta_ws_get_timing synchronises and returns the summed milliseconds and the
+ * number of launches of a class since timing was switched on or reset. */
+int bta_ws_set_timing(bta_ws_t ws, int on);
+int bta_ws_reset_timing(bta_ws_t ws);
+int bta_ws_get_timing(bta_ws_t ws, int kclass, double* total_ms, int64_t* count);
+
 /* In-place triangular sweep with the factor held in D, L, B, T (as left by
  * bta_potrf).  trans == 0 solves LL y = rhs (serinv.pobtas(..., trans="N")),
  * trans == 1 solves LL^T x = rhs (serinv.pobtas(..., trans="C")); the pair
@@ -96,6 +107,60 @@ int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
  * SerinvSolver.selected_inversion (structured_solver.py:168-190). */
 int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
               void* stream);
+
+/* ---------------------------------------------------------- assembly */
+
+/* Dense BTA blocks of Q_st = sum_{m,k} coef[m*nK+k] * (M_m (x) S_k) written
+ * straight into D (n,b,b) and L (n-1,b,b), every entry (zeros included).
+ * Spatial S_k share one union CSR pattern (rowptr[b+1], colidx[nnz]) with
+ * values vals[k*nnz + e] (nK <= 4); temporal M_m are tridiagonal, given by
+ * Mdiag[m*n + t] and Moff[m*(n-1) + t] (nM*nK <= 16).  If a > 0 the arrow
+ * blocks B are cleared and the tip T set to tip_diag * I (the fixed-effects
+ * prior).  Replaces the sparse Kronecker assembly of
+ * SpatioTemporalSubModel.construct_Q_prior (spatio_temporal.py:114-149), the
+ * regression tip (regression.py:28-34), Model.construct_Q_prior
+ * (model.py:306-382) and the sparse->BTA scatter that follows it
+ * (structured_solver.py:192-250). */
+int bta_assemble_kron(double* D, double* L, double* B, double* T, int n, int b, int a,
+                      const int* rowptr, const int* colidx, const double* vals, int nnz, int nK,
+                      const double* Mdiag, const double* Moff, int nM, const double* coef,
+                      double tip_diag, void* stream);
+
+/* Scatter COO values into the blocks: dst(code[e]) (+)= scale * val[src_idx[e]]
+ * (src_idx NULL: val[e]); code = (buffer << 60) | element offset with buffer
+ * 0..3 = D, L, B, T.  Destinations must be unique (no atomics; deterministic).
+ * accumulate != 0 adds (e.g. Q_cond = Q_prior + exp(theta_o) A^T A,
+ * model.py:384-423), else stores (the drop-in sparse scatter of
+ * SerinvSolver._spmatrix_to_bta, structured_solver.py:252-390, whose
+ * per-block index it precomputes once, as blockmapping.py:66-195 does). */
+int bta_scatter_coo(double* D, double* L, double* B, double* T, const uint64_t* code,
+                    const double* val, const int* src_idx, int64_t count, double scale,
+                    int accumulate, void* stream);
+
+/* out[e] = block entry at code[e]: the gather of the selected inverse on a
+ * sparsity pattern (SerinvSolver._structured_to_spmatrix,
+ * structured_solver.py:470-538). */
+int bta_gather_coo(const double* D, const double* L, const double* B, const double* T,
+                   const uint64_t* code, double* out, int64_t count, void* stream);
+
+/* out[0] = sum_e wval[e] * G(e), G = block entry at code[e] (mode 0) or
+ * x[grow[e]] * x[gcol[e]] (mode 1); deterministic two-pass reduction using
+ * work[nwork].  No reference counterpart: it evaluates tr(Sigma A^T A) and
+ * x^T A^T A x for the adjoint gradient of f(theta), whose finite-difference
+ * form is PyINLA._objective_function (pyinla.py:512-600). */
+int bta_bilinear_coo(const double* D, const double* L, const double* B, const double* T,
+                     const uint64_t* code, const int* grow, const int* gcol, const double* x,
+                     const double* wval, int64_t count, int mode, double* work, int nwork,
+                     double* out, void* stream);
+
+/* Kronecker contractions per block t and spatial matrix k (nK <= 4):
+ * mode 0: out[t*2nK + 2k] = <D_t, S_k>_F, out[t*2nK + 2k+1] = <L_t, S_k>_F;
+ * mode 1: x_t^T S_k x_t and x_{t+1}^T S_k x_t.  work holds
+ * n*ceil(b/64)*2nK doubles.  No reference counterpart (adjoint traces
+ * tr(Sigma dQ/dtheta) restricted to the BTA pattern). */
+int bta_kron_contract(const double* D, const double* L, const double* x, int n, int b,
+                      const int* rowptr, const int* colidx, const double* vals, int nnz, int nK,
+                      int mode, double* work, double* out, void* stream);
 
 /* Diagnostic (no reference counterpart): runs one v_mfma_f64_16x16x4f64 with
  * A[i][k] = i + 100k, B[k][j] = j + 1000k and writes each lane's four

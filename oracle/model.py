@@ -1,0 +1,146 @@
+"""CPU restatement of f(theta) and its finite-difference gradient (oracle).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Follows the reference's Gaussian path:
+  Q_st      sum of sparse Kronecker products (spatio_temporal.py:114-149,
+            here with an arbitrary coefficient table)
+  Q_prior   block-diag(Q_st, prec * I_nb) (model.py:306-382, regression.py:28-34)
+  Q_cond    Q_prior - A^T (-e^{theta_o} I) A (model.py:384-423, gaussian.py:98-131)
+  rhs       -Q_prior x + A^T grad_lik at x = 0, eta = 0 -> e^{theta_o} A^T y
+            (model.py:425-449, gaussian.py:64-96)
+  f         pyinla.py:602-698 with _evaluate_conditional_latent_parameters
+            (:1112-1167) and _evaluate_prior_latent_parameters (:1076-1110)
+  FD grad   pyinla.py:512-600
+The factorizations use the BTA oracle on blocks scattered as
+structured_solver.py:214-250 does.
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sps
+
+from . import bta as ob
+
+
+def q_st(spatial, temporal, table):
+    """sum_{m,k} table[m][k] kron(M_m, S_k) as scipy CSR."""
+    Q = None
+    for m, M in enumerate(temporal):
+        for k, S in enumerate(spatial):
+            if k >= table.shape[1] or table[m, k] == 0.0:
+                continue
+            term = table[m, k] * sps.kron(M, S, format="csr")
+            Q = term if Q is None else Q + term
+    return Q.tocsr()
+
+
+def q_prior(prob, table):
+    Qst = q_st(prob.spatial[: table.shape[1]], prob.temporal, table)
+    return sps.block_diag([Qst, prob.fixed_prec * sps.identity(prob.nb)], format="csr")
+
+
+def q_cond(prob, Qp, theta):
+    A = sps.csr_matrix(prob.A)
+    return (Qp + np.exp(theta[3]) * (A.T @ A)).tocsr()
+
+
+def scatter_bta(Q, n, b, a, sparsity="bta"):
+    """Dense BTA blocks of sparse Q (structured_solver.py:214-250)."""
+    Qc = sps.csc_matrix(Q)
+    D = np.zeros((n, b, b))
+    L = np.zeros((max(n - 1, 0), b, b))
+    B = np.zeros((n, a, b)) if sparsity == "bta" else None
+    T = np.zeros((a, a)) if sparsity == "bta" else None
+    for i in range(n):
+        D[i] = Qc[i * b:(i + 1) * b, i * b:(i + 1) * b].toarray()
+        if i < n - 1:
+            L[i] = Qc[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b].toarray()
+        if sparsity == "bta":
+            B[i] = Qc[n * b:, i * b:(i + 1) * b].toarray()
+    if sparsity == "bta":
+        T[:] = Qc[n * b:, n * b:].toarray()
+    return D, L, B, T
+
+
+def log_prior(theta, mean, prec):
+    return float(np.sum(-0.5 * prec * (np.asarray(theta) - mean) ** 2))
+
+
+def evaluate_f(prob, table_fn, theta, return_parts=False):
+    theta = np.asarray(theta, dtype=np.float64)
+    n, b, a = prob.nt, prob.ns, prob.nb
+    table = table_fn(theta)
+    Qp = q_prior(prob, table)
+    Qc = q_cond(prob, Qp, theta)
+    A = sps.csr_matrix(prob.A)
+    rhs = np.exp(theta[3]) * (A.T @ prob.y)
+    D, L, B, T = scatter_bta(Qc, n, b, a, "bta")
+    ob.pobtaf(D, L, B, T)
+    x = rhs.copy()
+    ob.pobtas(D, L, B, T, x, "N")
+    ob.pobtas(D, L, B, T, x, "C")
+    ld_c = ob.logdet(D, T)
+    quad = float(x @ (Qc @ x))
+    Dp, Lp, _, _ = scatter_bta(Qp, n, b, a, "bt")
+    ob.pobtf(Dp, Lp)
+    ld_p = ob.logdet(Dp)
+    logp = log_prior(theta, prob.prior_mean, prob.prior_prec)
+    m = A.shape[0]
+    lik = 0.5 * theta[3] * m - 0.5 * np.exp(theta[3]) * float(prob.y @ prob.y)
+    f = (0.5 * ld_c - 0.5 * quad) - (logp + lik + 0.5 * ld_p)
+    if return_parts:
+        return dict(f=f, ld_c=ld_c, ld_p=ld_p, quad=quad, logp=logp, lik=lik, x=x)
+    return f
+
+
+def fd_gradient(prob, table_fn, theta, eps=1e-3):
+    """Central FD with the reference's evaluation order (pyinla.py:540-580)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    d = theta.size
+    mat = np.repeat(theta[:, None], 2 * d + 1, axis=1)
+    mat[:, 1:1 + d] += eps * np.eye(d)
+    mat[:, d + 1:] -= eps * np.eye(d)
+    fv = np.zeros(2 * d + 1)
+    for k in range(2 * d, -1, -1):
+        fv[k] = evaluate_f(prob, table_fn, mat[:, k])
+    return fv[0], (fv[1:1 + d] - fv[d + 1:]) / (2 * eps)
+
+
+def dense_f_torch(prob, table_fn_torch, theta):
+    """f(theta) from dense float64 torch linear algebra (differentiable).
+
+    Independent of the BTA recurrences: dense Cholesky of the full matrices,
+    used to pin the adjoint gradient by autograd at small sizes.
+    """
+    import torch
+    th = theta
+    n, b, a = prob.nt, prob.ns, prob.nb
+    table = table_fn_torch(th)
+    Sd = [torch.as_tensor(s.toarray()) for s in prob.spatial[: table.shape[1]]]
+    Md = [torch.as_tensor(sps.csr_matrix(M).toarray()) for M in prob.temporal]
+    N = n * b + a
+    Qst = torch.zeros((n * b, n * b), dtype=torch.float64)
+    for m, M in enumerate(Md):
+        for k, S in enumerate(Sd):
+            Qst = Qst + table[m, k] * torch.kron(M, S)
+    Qp = torch.zeros((N, N), dtype=torch.float64)
+    Qp[: n * b, : n * b] = Qst
+    Qp[n * b:, n * b:] = prob.fixed_prec * torch.eye(a, dtype=torch.float64)
+    A = torch.as_tensor(sps.csr_matrix(prob.A).toarray())
+    y = torch.as_tensor(prob.y)
+    eo = torch.exp(th[3])
+    Qc = Qp + eo * (A.T @ A)
+    rhs = eo * (A.T @ y)
+    Lc = torch.linalg.cholesky(Qc)
+    x = torch.cholesky_solve(rhs[:, None], Lc)[:, 0]
+    ld_c = 2 * torch.log(torch.diagonal(Lc)).sum()
+    Lp = torch.linalg.cholesky(Qst)
+    ld_p = 2 * torch.log(torch.diagonal(Lp)).sum()
+    quad = x @ (Qc @ x)
+    mean = torch.as_tensor(prob.prior_mean)
+    prec = torch.as_tensor(prob.prior_prec)
+    logp = torch.sum(-0.5 * prec * (th - mean) ** 2)
+    m = A.shape[0]
+    lik = 0.5 * th[3] * m - 0.5 * eo * (y @ y)
+    return (0.5 * ld_c - 0.5 * quad) - (logp + lik + 0.5 * ld_p)

@@ -21,11 +21,34 @@ struct GraphKey {
 struct bta_ws {
   bta::Workspace w;
   int use_graph = 0;
+  int saved_use_graph = 0;
   hipStream_t gstream = nullptr;  // capture / replay stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   std::map<GraphKey, hipGraphExec_t> graphs;
   std::mutex mu;
 };
+
+namespace bta {
+
+hipEvent_t TimingLog::begin(int cls, hipStream_t st) {
+  auto& v = ev[cls];
+  if (used[cls] == v.size()) {
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    v.emplace_back(a, b);
+  }
+  hipEvent_t e = v[used[cls]].first;
+  hipEventRecord(e, st);
+  return e;
+}
+
+void TimingLog::end(int cls, hipStream_t st) {
+  hipEventRecord(ev[cls][used[cls]].second, st);
+  used[cls] += 1;
+}
+
+}  // namespace bta
 
 namespace {
 
@@ -133,6 +156,7 @@ int bta_ws_destroy(bta_ws_t ws) {
   if (!ws) return 0;
   hipSetDevice(ws->w.device);
   hipDeviceSynchronize();
+  if (ws->w.tlog) bta_ws_set_timing(ws, 0);
   for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);
   if (ws->gstream) hipStreamDestroy(ws->gstream);
   if (ws->ev_in) hipEventDestroy(ws->ev_in);
@@ -222,6 +246,50 @@ int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* st
 }
 
 double* bta_ws_logdet_ptr(bta_ws_t ws) { return ws ? ws->w.logdet : nullptr; }
+
+int bta_ws_set_timing(bta_ws_t ws, int on) {
+  if (!ws) return BTA_E_ARG;
+  if (on) {
+    if (!ws->w.tlog) ws->w.tlog = new bta::TimingLog();
+    for (int c = 0; c < bta::KC_COUNT; ++c) ws->w.tlog->used[c] = 0;
+    ws->saved_use_graph = ws->use_graph;
+    ws->use_graph = 0;  // events bracket individual launches, so no capture
+  } else if (ws->w.tlog) {
+    for (int c = 0; c < bta::KC_COUNT; ++c)
+      for (auto& pr : ws->w.tlog->ev[c]) {
+        hipEventDestroy(pr.first);
+        hipEventDestroy(pr.second);
+      }
+    delete ws->w.tlog;
+    ws->w.tlog = nullptr;
+    ws->use_graph = ws->saved_use_graph;
+  }
+  return 0;
+}
+
+int bta_ws_get_timing(bta_ws_t ws, int kclass, double* total_ms, int64_t* count) {
+  if (!ws || kclass < 0 || kclass >= bta::KC_COUNT || !ws->w.tlog) return BTA_E_ARG;
+  bta::TimingLog* t = ws->w.tlog;
+  double tot = 0.0;
+  for (size_t q = 0; q < t->used[kclass]; ++q) {
+    hipError_t e = hipEventSynchronize(t->ev[kclass][q].second);
+    if (e != hipSuccess) return (int)e;
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, t->ev[kclass][q].first, t->ev[kclass][q].second);
+    if (e != hipSuccess) return (int)e;
+    tot += ms;
+  }
+  if (total_ms) *total_ms = tot;
+  if (count) *count = (int64_t)t->used[kclass];
+  return 0;
+}
+
+int bta_ws_reset_timing(bta_ws_t ws) {
+  if (!ws || !ws->w.tlog) return BTA_E_ARG;
+  hipDeviceSynchronize();
+  for (int c = 0; c < bta::KC_COUNT; ++c) ws->w.tlog->used[c] = 0;
+  return 0;
+}
 
 int bta_selftest_mfma(double* out, void* stream) {
   if (!out) return BTA_E_ARG;

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <memory>
+#include <utility>
+#include <vector>
 
 #include "bta_device.h"
 #include "bta_gemm.h"
@@ -10,6 +12,24 @@
 namespace bta {
 
 static constexpr unsigned long long kInfoNone = ~0ull;
+
+// Kernel classes for the optional event timing of bta_ws_set_timing.
+enum KernelClass : int {
+  KC_PANEL = 0,     // k_panel_step (tile-column factorization)
+  KC_UPDATE = 1,    // k_gemm_nt_sub (block updates D_{i+1}, B_{i+1}, T)
+  KC_GROUPED = 2,   // k_gemm_grouped (block inverses, selected inversion)
+  KC_GEMV = 3,      // k_gemv (triangular sweeps)
+  KC_OTHER = 4,
+  KC_COUNT = 5
+};
+
+// Event pairs recorded around launches while timing is enabled.
+struct TimingLog {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[KC_COUNT];
+  size_t used[KC_COUNT] = {0, 0, 0, 0, 0};
+  hipEvent_t begin(int cls, hipStream_t st);
+  void end(int cls, hipStream_t st);
+};
 
 // Per-solver device workspace.  Allocated once (never inside a hot call) for a
 // fixed (n, b, a); every enqueue function only reads and writes it.
@@ -37,6 +57,20 @@ struct Workspace {
   const void* trtri_key = nullptr;
   const void* potri_key[4] = {nullptr, nullptr, nullptr, nullptr};
   int potri_a = -1;
+  TimingLog* tlog = nullptr;  // non-null: bracket launches with events
+};
+
+// RAII bracket: records a start event now and the stop event at scope exit.
+struct TimedScope {
+  Workspace& ws;
+  int cls;
+  hipStream_t st;
+  TimedScope(Workspace& w, int c, hipStream_t s) : ws(w), cls(c), st(s) {
+    if (ws.tlog) ws.tlog->begin(cls, st);
+  }
+  ~TimedScope() {
+    if (ws.tlog) ws.tlog->end(cls, st);
+  }
 };
 
 int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,

@@ -338,6 +338,7 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     p.nTa = p.B ? nTa : 0;
     for (int k = 0; k < nT; ++k) {
       p.k = k;
+      TimedScope ts(ws, KC_PANEL, st);
       hipLaunchKernelGGL(k_panel_step, dim3(panel_grid(nT, p.nTL, p.nTa, k)), dim3(BTA_THREADS), 0, st, p);
     }
     UpdJobs jobs;
@@ -351,6 +352,7 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
       }
     }
     if (a > 0) jobs.job[jobs.njobs++] = UpdJob{T, p.B, p.B, a, a, b, a, b, b, 1, 0, 0};
+    TimedScope ts(ws, KC_UPDATE, st);
     launch_gemm_jobs(jobs, st);
   }
   int npart = n * nT;

@@ -84,7 +84,10 @@ int trtri_prepare(Workspace& ws, const double* D, int n, int b) {
 int trtri_enqueue(Workspace& ws, const double* D, int n, int b, hipStream_t st) {
   const int nT = (b + 63) / 64;
   hipLaunchKernelGGL(k_leaf_copy, dim3(n * nT), dim3(BTA_THREADS), 0, st, ws.wfull, ws.winv, b, nT);
-  for (size_t q = 0; q < ws.trtri_plan->launches.size(); ++q) ws.trtri_plan->launch((int)q, st);
+  for (size_t q = 0; q < ws.trtri_plan->launches.size(); ++q) {
+    TimedScope ts(ws, KC_GROUPED, st);
+    ws.trtri_plan->launch((int)q, st);
+  }
   hipLaunchKernelGGL(k_zero_upper, dim3(n * nT), dim3(BTA_THREADS), 0, st, ws.wfull, b, nT);
   return (int)hipGetLastError();
 }
@@ -164,7 +167,10 @@ int potri_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     const int rc = trtri_enqueue(ws, D, n, b, st);
     if (rc) return rc;
   }
-  for (size_t q = 0; q < ws.potri_plan->launches.size(); ++q) ws.potri_plan->launch((int)q, st);
+  for (size_t q = 0; q < ws.potri_plan->launches.size(); ++q) {
+    TimedScope ts(ws, KC_GROUPED, st);
+    ws.potri_plan->launch((int)q, st);
+  }
   return (int)hipGetLastError();
 }
 

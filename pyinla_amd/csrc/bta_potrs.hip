@@ -65,29 +65,29 @@ int potrs_enqueue(Workspace& ws, const double* D, const double* L, const double*
         const double* xp = rhs + (int64_t)(i - 1) * b * ldrhs;
         GemvJob g = gjob(xi, b, ldrhs, nrhs, 1.0);
         g.seg[g.nseg++] = gseg(L + (int64_t)(i - 1) * b * b, b, 0, xp, ldrhs, b, -1.0);
-        gemv_launch(g, 0, st);
+        { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(g, 0, st); }
       }
       GemvJob w = gjob(tmp, b, nrhs, nrhs, 0.0);
       w.seg[w.nseg++] = gseg(ws.wfull + (int64_t)i * b * b, b, 0, xi, ldrhs, b, 1.0, 1);
-      gemv_launch(w, 0, st);
+      { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(w, 0, st); }
       copy_rows(xi, ldrhs, tmp, nrhs, b, nrhs, st);
       if (a > 0) {
         GemvJob ga = gjob(xa, a, ldrhs, nrhs, 1.0);
         ga.seg[ga.nseg++] = gseg(B + (int64_t)i * a * b, b, 0, xi, ldrhs, b, -1.0);
-        gemv_launch(ga, 0, st);
+        { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(ga, 0, st); }
       }
     }
     if (a > 0) {
       GemvJob t = gjob(tmp, a, nrhs, nrhs, 0.0);
       t.seg[t.nseg++] = gseg(ws.tinv, 64, 0, xa, ldrhs, a, 1.0, 1);
-      gemv_launch(t, 0, st);
+      { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(t, 0, st); }
       copy_rows(xa, ldrhs, tmp, nrhs, a, nrhs, st);
     }
   } else {
     if (a > 0) {
       GemvJob t = gjob(tmp, a, nrhs, nrhs, 0.0);
       t.seg[t.nseg++] = gseg(ws.tinv, 64, 1, xa, ldrhs, a, 1.0, 2);
-      gemv_launch(t, 1, st);
+      { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(t, 1, st); }
       copy_rows(xa, ldrhs, tmp, nrhs, a, nrhs, st);
     }
     for (int i = n - 1; i >= 0; --i) {
@@ -96,10 +96,10 @@ int potrs_enqueue(Workspace& ws, const double* D, const double* L, const double*
       if (i < n - 1)
         g.seg[g.nseg++] = gseg(L + (int64_t)i * b * b, b, 1, rhs + (int64_t)(i + 1) * b * ldrhs, ldrhs, b, -1.0);
       if (a > 0) g.seg[g.nseg++] = gseg(B + (int64_t)i * a * b, b, 1, xa, ldrhs, a, -1.0);
-      if (g.nseg > 0) gemv_launch(g, 1, st);
+      if (g.nseg > 0) { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(g, 1, st); }
       GemvJob w = gjob(tmp, b, nrhs, nrhs, 0.0);
       w.seg[w.nseg++] = gseg(ws.wfull + (int64_t)i * b * b, b, 1, xi, ldrhs, b, 1.0, 2);
-      gemv_launch(w, 1, st);
+      { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(w, 1, st); }
       copy_rows(xi, ldrhs, tmp, nrhs, b, nrhs, st);
     }
   }

@@ -214,5 +214,104 @@ class SerinvSolver:
         self.total_bytes += nbytes
         return self.total_bytes
 
+    # ------------------------------------------------- sparse <-> blocks
+
+    def _pattern_codes(self, rows, cols, sparsity: str, lower_only: bool):
+        """Select the entries that the reference's scatter places in the blocks
+        and return (entry indices, BTA codes).
+
+        Placed (structured_solver.py:214-250): both triangles of the diagonal
+        blocks, the sub-diagonal blocks (i+1, i), and for "bta" the arrow rows
+        (n, i) and the tip.  lower_only keeps i >= j only (the gather of
+        structured_solver.py:470-538 works on tril(A)).
+        """
+        from .st_model import bta_codes
+        n, b = self.n_diag_blocks, self.diagonal_blocksize
+        a = self.arrowhead_blocksize if sparsity == "bta" else 0
+        nb_ = n * b
+        rows = np.asarray(rows, dtype=np.int64)
+        cols = np.asarray(cols, dtype=np.int64)
+        st_r, st_c = rows < nb_, cols < nb_
+        bi = np.where(st_r, rows // b, n)
+        bj = np.where(st_c, cols // b, n)
+        keepD = st_r & st_c & (bi == bj)
+        keepL = st_r & st_c & (bi == bj + 1)
+        keep = keepD | keepL
+        if a > 0:
+            keep |= (~st_r) & (rows < nb_ + a) & ((cols < nb_) | (cols < nb_ + a))
+        if lower_only:
+            keep &= rows >= cols
+        idx = np.nonzero(keep)[0]
+        r, c = rows[idx], cols[idx]
+        # entries of the upper triangle of a diagonal block are stored as they
+        # are (the blocks hold both triangles, as in the reference)
+        upper = (r < c)
+        codes = np.empty(len(idx), dtype=np.uint64)
+        if (~upper).any():
+            codes[~upper] = bta_codes(r[~upper], c[~upper], n, b, a)
+        if upper.any():
+            ru, cu = r[upper], c[upper]
+            tip = (ru >= nb_)
+            cd = np.empty(len(ru), dtype=np.uint64)
+            if (~tip).any():  # inside a diagonal block: offset of (row, col) within D
+                blk = ru[~tip] // b
+                off = blk * b * b + (ru[~tip] % b) * b + (cu[~tip] % b)
+                cd[~tip] = off.astype(np.uint64)
+            if tip.any():
+                off = (ru[tip] - nb_) * a + (cu[tip] - nb_)
+                cd[tip] = (np.uint64(3) << np.uint64(60)) | off.astype(np.uint64)
+            codes[upper] = cd
+        return idx, codes
+
     def _spmatrix_to_structured(self, A, sparsity: str) -> None:
-        raise NotImplementedError
+        """Zero the blocks and scatter the sparse matrix into them
+        (structured_solver.py:192-250).  ``A`` is a scipy sparse matrix or a
+        torch sparse tensor.  The scatter map is computed once per sparsity
+        pattern and cached, as the reference's GPU path does (:258-365): the
+        pattern and COO order must stay fixed between calls."""
+        import scipy.sparse as sps
+        _check_sparsity(sparsity)
+        if isinstance(A, torch.Tensor):
+            Ac = A.coalesce().cpu() if A.layout == torch.sparse_coo else A.to_sparse_coo().coalesce().cpu()
+            ind = Ac.indices().numpy()
+            rows, cols, data = ind[0], ind[1], Ac.values().numpy().astype(np.float64)
+        else:
+            coo = sps.coo_matrix(A)
+            rows, cols, data = coo.row, coo.col, np.asarray(coo.data, dtype=np.float64)
+        key = (sparsity, len(data), rows[:4].tobytes(), cols[-4:].tobytes())
+        ent = self._scatter_cache.get(key)
+        if ent is None:
+            idx, codes = self._pattern_codes(rows, cols, sparsity, lower_only=False)
+            ent = (torch.from_numpy(idx.astype(np.int32)).to(self.device),
+                   torch.from_numpy(codes.view(np.int64)).to(self.device), len(idx))
+            self._scatter_cache[key] = ent
+            self.total_bytes += ent[0].numel() * 4 + ent[1].numel() * 8
+        src_idx, codes, count = ent
+        for t in (self.A_diagonal_blocks, self.A_lower_diagonal_blocks, self.A_arrow_bottom_blocks,
+                  self.A_arrow_tip_block):
+            if t is not None:
+                t.zero_()
+        vals = torch.from_numpy(np.ascontiguousarray(data)).to(self.device)
+        D, L, B, T, a = self._blocks(sparsity)
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.bta_scatter_coo(_ptr(D), _ptr(L), _ptr(B), _ptr(T), _ptr(codes), _ptr(vals),
+                                                _ptr(src_idx), count, 1.0, 0, _stream_ptr(self.device)),
+                       "bta_scatter_coo")
+
+    def _structured_to_spmatrix(self, A, sparsity: str):
+        """Selected inverse on the lower pattern of ``A``, symmetrised
+        (structured_solver.py:470-538).  Returns a scipy CSC matrix."""
+        import scipy.sparse as sps
+        _check_sparsity(sparsity)
+        Bl = sps.coo_matrix(sps.tril(sps.csc_matrix(A)))
+        idx, codes = self._pattern_codes(Bl.row, Bl.col, sparsity, lower_only=True)
+        rows, cols = Bl.row[idx], Bl.col[idx]
+        out = torch.empty(len(idx), dtype=torch.float64, device=self.device)
+        codes_t = torch.from_numpy(codes.view(np.int64)).to(self.device)
+        D, L, B, T, a = self._blocks(sparsity)
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.bta_gather_coo(_ptr(D), _ptr(L), _ptr(B), _ptr(T), _ptr(codes_t), _ptr(out),
+                                               len(idx), _stream_ptr(self.device)), "bta_gather_coo")
+        data = out.cpu().numpy()
+        B_out = sps.coo_matrix((data, (rows, cols)), shape=Bl.shape).tocsc()
+        return B_out + sps.tril(B_out, k=-1).T

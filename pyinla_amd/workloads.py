@@ -133,3 +133,108 @@ def _q_from_factor_torch(Ld, Ll, Lb, Lt):
             QL[i] = Ll[i] @ Ld[i].T.contiguous()
         QT += Lb[i] @ Lb[i].T.contiguous()
     return QD, QL, QB, QT
+
+
+# ------------------------------------------------------- spatio-temporal model
+
+@dataclass
+class STProblem:
+    """Inputs of the Gaussian spatio-temporal model on a regular grid.
+
+    spatial  : [c0, g1, g2] (scipy CSR, ns x ns)
+    temporal : [M0, M1, M2] (scipy CSR, nt x nt, tridiagonal)
+    A        : observation matrix (scipy CSR, m x (nt*ns + nb)); latent order
+               is t-major space-time (index t*ns + s) then the fixed effects,
+               the reference's kron(M, S) ordering with the regression last.
+    y        : observations (m,)
+    theta    : evaluation point, theta = (theta_s, theta_t, theta_e, theta_o)
+    """
+    ns: int
+    nt: int
+    nb: int
+    spatial: list
+    temporal: list
+    A: object
+    y: np.ndarray
+    theta: np.ndarray
+    prior_mean: np.ndarray
+    prior_prec: np.ndarray
+    fixed_prec: float
+
+
+def grid_spatial_matrices(nx: int, ny: int, hx: float, hy: float | None = None):
+    """c0 = h^2 I, g1 = 5-point graph Laplacian, g2 = g1 c0^-1 g1 on an nx x ny grid.
+
+    Node index s = ix * ny + iy.  With hy == hx (or None) h^2 = hx^2, else
+    h^2 = hx * hy (the cell area).
+    """
+    import scipy.sparse as sps
+    hy = hx if hy is None else hy
+    ns = nx * ny
+    idx = np.arange(ns).reshape(nx, ny)
+    rows, cols = [], []
+    for a, b_ in ((idx[:-1, :], idx[1:, :]), (idx[:, :-1], idx[:, 1:])):
+        rows.append(a.ravel())
+        cols.append(b_.ravel())
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    adj = sps.coo_matrix((np.ones(2 * len(r)), (np.concatenate([r, c]), np.concatenate([c, r]))), shape=(ns, ns)).tocsr()
+    deg = np.asarray(adj.sum(axis=1)).ravel()
+    g1 = (sps.diags(deg) - adj).tocsr()
+    area = hx * hy
+    c0 = (area * sps.identity(ns, format="csr")).tocsr()
+    g2 = (g1 @ sps.diags(np.full(ns, 1.0 / area)) @ g1).tocsr()
+    for m in (c0, g1, g2):
+        m.sort_indices()
+    return [c0, g1, g2]
+
+
+def temporal_matrices(nt: int):
+    """M0 = I, M1 = first-difference Gram (path Laplacian), M2 = tridiag(-1, 2, -1)."""
+    import scipy.sparse as sps
+    e = np.ones(nt)
+    M0 = sps.identity(nt, format="csr")
+    d1 = np.full(nt, 2.0)
+    if nt > 1:
+        d1[0] = d1[-1] = 1.0
+    else:
+        d1[:] = 0.0
+    M1 = sps.diags([-e[:-1], d1, -e[:-1]], [-1, 0, 1], format="csr")
+    M2 = sps.diags([-e[:-1], 2.0 * e, -e[:-1]], [-1, 0, 1], format="csr")
+    return [M0.tocsr(), M1.tocsr(), M2.tocsr()]
+
+
+def st_problem(nt: int = 100, nx: int = 40, ny: int = 40, nb: int = 6, obs_per_node: int = 4,
+               seed: int = 1, hx: float | None = None, hy: float | None = None) -> STProblem:
+    """Synthetic Gaussian spatio-temporal model (grid mesh, random observations).
+
+    Draw order: (1) observation nodes: m = obs_per_node * ns * nt integers in
+    [0, ns*nt); (2) covariates N(0,1), shape (m, nb); (3) latent space-time
+    field N(0, 0.5), shape (ns*nt,); (4) fixed effects N(0,1), shape (nb,);
+    (5) noise N(0,1) * exp(-theta_o/2), shape (m,); (6) theta perturbation
+    N(0, 0.1), shape (4,), added to (0, 0, 0, 1).
+    Defaults: the 40 x 40 grid (h = 1/39), nt = 100, nb = 6 case.
+    """
+    import scipy.sparse as sps
+    ns = nx * ny
+    hx = 1.0 / (nx - 1) if hx is None else hx
+    hy = (1.0 / (ny - 1) if hy is None else hy) if (hy is not None or nx != ny) else hx
+    spatial = grid_spatial_matrices(nx, ny, hx, hy)
+    temporal = temporal_matrices(nt)
+    rng = np.random.default_rng(seed)
+    m = obs_per_node * ns * nt
+    nodes = rng.integers(0, ns * nt, size=m)
+    X = rng.normal(0.0, 1.0, (m, nb))
+    u = rng.normal(0.0, 0.5, ns * nt)
+    beta = rng.normal(0.0, 1.0, nb)
+    theta_o = 1.0
+    noise = rng.normal(0.0, 1.0, m) * np.exp(-0.5 * theta_o)
+    theta = np.array([0.0, 0.0, 0.0, 1.0]) + rng.normal(0.0, 0.1, 4)
+    N = ns * nt + nb
+    rows = np.repeat(np.arange(m), 1 + nb)
+    cols = np.concatenate([nodes[:, None], ns * nt + np.arange(nb)[None, :].repeat(m, 0)], axis=1).ravel()
+    vals = np.concatenate([np.ones((m, 1)), X], axis=1).ravel()
+    A = sps.csr_matrix((vals, (rows, cols)), shape=(m, N))
+    y = u[nodes] + X @ beta + noise
+    return STProblem(ns=ns, nt=nt, nb=nb, spatial=spatial, temporal=temporal, A=A, y=y, theta=theta,
+                     prior_mean=np.array([0.0, 0.0, 0.0, 1.0]), prior_prec=np.full(4, 0.5), fixed_prec=1e-3)
