@@ -65,6 +65,21 @@ size_t bta_ws_bytes(int n, int b, int a);
 int bta_potrf(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
               void* stream);
 
+/* Batched factorization of independent problems that may differ in (n, b, a)
+ * (the 2 dim(theta) + 1 finite-difference evaluations, prior / conditional
+ * pairs, mixed-size batches).  Problem q uses workspace ws[q] (created for its
+ * n, b and an arrowhead >= a[q]) and blocks D[q], L[q], B[q], T[q]; a[q] == 0
+ * factors the BT part only.  Every block step / tile column is one launch over
+ * all problems that have it, and bta_batch_potrf replays the whole sequence as
+ * one hipGraph (use_graph != 0).  Results (logdet, info) are read per problem
+ * with bta_ws_status.  No reference counterpart in one process: the reference
+ * spreads these evaluations over MPI ranks (pyinla.py:93-106,541-572). */
+typedef struct bta_batch* bta_batch_t;
+int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B, double** T,
+                     const int* a, int use_graph, bta_batch_t* out);
+int bta_batch_potrf(bta_batch_t batch, void* stream);
+int bta_batch_destroy(bta_batch_t batch);
+
 /* Synchronise the stream and return the status of the last bta_potrf:
  *   info   : 0 if positive definite, else 1 + global row of the first failing
  *            pivot (LAPACK convention);
@@ -110,8 +125,11 @@ int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
 
 /* ---------------------------------------------------------- assembly */
 
-/* Dense BTA blocks of Q_st = sum_{m,k} coef[m*nK+k] * (M_m (x) S_k) written
- * straight into D (n,b,b) and L (n-1,b,b), every entry (zeros included).
+/* Dense BTA blocks of Q = sum_{v,w,m,k} coef[((v*nv+w)*nM+m)*nK+k] *
+ * (M_m (x) E_vw (x) S_k) written straight into D (n,b,b) and L (n-1,b,b),
+ * every entry (zeros included); nv fields (nv <= 4, b = nv*ns) are interleaved
+ * per time block as the coregional model orders them
+ * (coregional_model.py:754-786); nv = 1 is the single-field case.
  * Spatial S_k share one union CSR pattern (rowptr[b+1], colidx[nnz]) with
  * values vals[k*nnz + e] (nK <= 4); temporal M_m are tridiagonal, given by
  * Mdiag[m*n + t] and Moff[m*(n-1) + t] (nM*nK <= 16).  If a > 0 the arrow
@@ -121,7 +139,7 @@ int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
  * regression tip (regression.py:28-34), Model.construct_Q_prior
  * (model.py:306-382) and the sparse->BTA scatter that follows it
  * (structured_solver.py:192-250). */
-int bta_assemble_kron(double* D, double* L, double* B, double* T, int n, int b, int a,
+int bta_assemble_kron(double* D, double* L, double* B, double* T, int n, int b, int a, int nv,
                       const int* rowptr, const int* colidx, const double* vals, int nnz, int nK,
                       const double* Mdiag, const double* Moff, int nM, const double* coef,
                       double tip_diag, void* stream);
@@ -153,12 +171,13 @@ int bta_bilinear_coo(const double* D, const double* L, const double* B, const do
                      const double* wval, int64_t count, int mode, double* work, int nwork,
                      double* out, void* stream);
 
-/* Kronecker contractions per block t and spatial matrix k (nK <= 4):
- * mode 0: out[t*2nK + 2k] = <D_t, S_k>_F, out[t*2nK + 2k+1] = <L_t, S_k>_F;
- * mode 1: x_t^T S_k x_t and x_{t+1}^T S_k x_t.  work holds
- * n*ceil(b/64)*2nK doubles.  No reference counterpart (adjoint traces
- * tr(Sigma dQ/dtheta) restricted to the BTA pattern). */
-int bta_kron_contract(const double* D, const double* L, const double* x, int n, int b,
+/* Kronecker contractions per block t, field pair (v, w) and spatial matrix k
+ * (nK <= 4, nv <= 4, b = nv*ns): with q = ((t*nv + v)*nv + w)*nK + k,
+ * mode 0: out[2q] = <D_t[v,w], S_k>_F, out[2q+1] = <L_t[v,w], S_k>_F;
+ * mode 1: x_{t,v}^T S_k x_{t,w} and x_{t+1,v}^T S_k x_{t,w}.  work holds
+ * n*nv*ceil(ns/64)*2*nv*nK doubles.  No reference counterpart (adjoint
+ * traces tr(Sigma dQ/dtheta) restricted to the BTA pattern). */
+int bta_kron_contract(const double* D, const double* L, const double* x, int n, int b, int nv,
                       const int* rowptr, const int* colidx, const double* vals, int nnz, int nK,
                       int mode, double* work, double* out, void* stream);
 

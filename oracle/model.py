@@ -24,25 +24,45 @@ from . import bta as ob
 
 
 def q_st(spatial, temporal, table):
-    """sum_{m,k} table[m][k] kron(M_m, S_k) as scipy CSR."""
+    """sum_{m,k} table[m][k] kron(M_m, S_k) as scipy CSR; with a 4-D table
+    (nv, nv, nM, nK) the fields are interleaved per time block:
+    sum_{v,w,m,k} table[v][w][m][k] kron(M_m, kron(E_vw, S_k))."""
+    table = np.asarray(table)
+    if table.ndim == 2:
+        table = table[None, None]
+    nv = table.shape[0]
     Q = None
-    for m, M in enumerate(temporal):
-        for k, S in enumerate(spatial):
-            if k >= table.shape[1] or table[m, k] == 0.0:
-                continue
-            term = table[m, k] * sps.kron(M, S, format="csr")
-            Q = term if Q is None else Q + term
+    for v in range(nv):
+        for w in range(nv):
+            E = sps.csr_matrix(([1.0], ([v], [w])), shape=(nv, nv))
+            for m, M in enumerate(temporal):
+                for k, S in enumerate(spatial):
+                    if k >= table.shape[3] or table[v, w, m, k] == 0.0:
+                        continue
+                    term = table[v, w, m, k] * sps.kron(M, sps.kron(E, S), format="csr")
+                    Q = term if Q is None else Q + term
     return Q.tocsr()
 
 
+def _obs(prob):
+    """[(A_v, y_v, theta index of its noise precision)] (one set for nv = 1)."""
+    if getattr(prob, "A_list", None) is not None:
+        d_q = len(prob.theta) - len(prob.A_list)
+        return [(sps.csr_matrix(A), y, d_q + v) for v, (A, y) in enumerate(zip(prob.A_list, prob.y_list))]
+    return [(sps.csr_matrix(prob.A), prob.y, 3)]
+
+
 def q_prior(prob, table):
-    Qst = q_st(prob.spatial[: table.shape[1]], prob.temporal, table)
+    nK = np.asarray(table).shape[-1]
+    Qst = q_st(prob.spatial[:nK], prob.temporal, table)
     return sps.block_diag([Qst, prob.fixed_prec * sps.identity(prob.nb)], format="csr")
 
 
 def q_cond(prob, Qp, theta):
-    A = sps.csr_matrix(prob.A)
-    return (Qp + np.exp(theta[3]) * (A.T @ A)).tocsr()
+    Q = Qp
+    for A, y, idx in _obs(prob):
+        Q = Q + np.exp(theta[idx]) * (A.T @ A)
+    return Q.tocsr()
 
 
 def scatter_bta(Q, n, b, a, sparsity="bta"):
@@ -69,12 +89,12 @@ def log_prior(theta, mean, prec):
 
 def evaluate_f(prob, table_fn, theta, return_parts=False):
     theta = np.asarray(theta, dtype=np.float64)
-    n, b, a = prob.nt, prob.ns, prob.nb
+    n, b, a = prob.nt, getattr(prob, "nv", 1) * prob.ns, prob.nb
     table = table_fn(theta)
     Qp = q_prior(prob, table)
     Qc = q_cond(prob, Qp, theta)
-    A = sps.csr_matrix(prob.A)
-    rhs = np.exp(theta[3]) * (A.T @ prob.y)
+    obs = _obs(prob)
+    rhs = sum(np.exp(theta[idx]) * (A.T @ y) for A, y, idx in obs)
     D, L, B, T = scatter_bta(Qc, n, b, a, "bta")
     ob.pobtaf(D, L, B, T)
     x = rhs.copy()
@@ -86,8 +106,7 @@ def evaluate_f(prob, table_fn, theta, return_parts=False):
     ob.pobtf(Dp, Lp)
     ld_p = ob.logdet(Dp)
     logp = log_prior(theta, prob.prior_mean, prob.prior_prec)
-    m = A.shape[0]
-    lik = 0.5 * theta[3] * m - 0.5 * np.exp(theta[3]) * float(prob.y @ prob.y)
+    lik = sum(0.5 * theta[idx] * A.shape[0] - 0.5 * np.exp(theta[idx]) * float(y @ y) for A, y, idx in obs)
     f = (0.5 * ld_c - 0.5 * quad) - (logp + lik + 0.5 * ld_p)
     if return_parts:
         return dict(f=f, ld_c=ld_c, ld_p=ld_p, quad=quad, logp=logp, lik=lik, x=x)
@@ -111,27 +130,41 @@ def dense_f_torch(prob, table_fn_torch, theta):
     """f(theta) from dense float64 torch linear algebra (differentiable).
 
     Independent of the BTA recurrences: dense Cholesky of the full matrices,
-    used to pin the adjoint gradient by autograd at small sizes.
+    used to pin the adjoint gradient by autograd at small sizes.  table_fn_torch
+    returns a (nM, nK) table or an (nv, nv, nM, nK) coregional table.
     """
     import torch
     th = theta
-    n, b, a = prob.nt, prob.ns, prob.nb
+    nv = getattr(prob, "nv", 1)
+    n, b, a = prob.nt, nv * prob.ns, prob.nb
     table = table_fn_torch(th)
-    Sd = [torch.as_tensor(s.toarray()) for s in prob.spatial[: table.shape[1]]]
+    if table.dim() == 2:
+        table = table[None, None]
+    nK = table.shape[3]
+    Sd = [torch.as_tensor(s.toarray()) for s in prob.spatial[:nK]]
     Md = [torch.as_tensor(sps.csr_matrix(M).toarray()) for M in prob.temporal]
     N = n * b + a
     Qst = torch.zeros((n * b, n * b), dtype=torch.float64)
-    for m, M in enumerate(Md):
-        for k, S in enumerate(Sd):
-            Qst = Qst + table[m, k] * torch.kron(M, S)
+    for v in range(nv):
+        for w in range(nv):
+            E = torch.zeros((nv, nv), dtype=torch.float64)
+            E[v, w] = 1.0
+            for m, M in enumerate(Md):
+                for k, S in enumerate(Sd):
+                    Qst = Qst + table[v, w, m, k] * torch.kron(M, torch.kron(E, S))
     Qp = torch.zeros((N, N), dtype=torch.float64)
     Qp[: n * b, : n * b] = Qst
     Qp[n * b:, n * b:] = prob.fixed_prec * torch.eye(a, dtype=torch.float64)
-    A = torch.as_tensor(sps.csr_matrix(prob.A).toarray())
-    y = torch.as_tensor(prob.y)
-    eo = torch.exp(th[3])
-    Qc = Qp + eo * (A.T @ A)
-    rhs = eo * (A.T @ y)
+    Qc = Qp
+    rhs = torch.zeros(N, dtype=torch.float64)
+    lik = torch.zeros((), dtype=torch.float64)
+    for A_s, y_n, idx in _obs(prob):
+        A = torch.as_tensor(A_s.toarray())
+        y = torch.as_tensor(np.asarray(y_n))
+        eo = torch.exp(th[idx])
+        Qc = Qc + eo * (A.T @ A)
+        rhs = rhs + eo * (A.T @ y)
+        lik = lik + 0.5 * th[idx] * A.shape[0] - 0.5 * eo * (y @ y)
     Lc = torch.linalg.cholesky(Qc)
     x = torch.cholesky_solve(rhs[:, None], Lc)[:, 0]
     ld_c = 2 * torch.log(torch.diagonal(Lc)).sum()
@@ -141,6 +174,4 @@ def dense_f_torch(prob, table_fn_torch, theta):
     mean = torch.as_tensor(prob.prior_mean)
     prec = torch.as_tensor(prob.prior_prec)
     logp = torch.sum(-0.5 * prec * (th - mean) ** 2)
-    m = A.shape[0]
-    lik = 0.5 * th[3] * m - 0.5 * eo * (y @ y)
     return (0.5 * ld_c - 0.5 * quad) - (logp + lik + 0.5 * ld_p)

@@ -24,16 +24,19 @@ SIGNATURES = {
     "bta_ws_status": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(_D), C.POINTER(_I), _P]),
     "bta_ws_logdet_ptr": (_P, [_P]),
     "bta_selftest_mfma": (_I, [_P, _P]),
+    "bta_batch_create": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, C.POINTER(_P)]),
+    "bta_batch_potrf": (_I, [_P, _P]),
+    "bta_batch_destroy": (_I, [_P]),
     "bta_ws_set_timing": (_I, [_P, _I]),
     "bta_ws_reset_timing": (_I, [_P]),
     "bta_ws_get_timing": (_I, [_P, _I, C.POINTER(_D), C.POINTER(C.c_int64)]),
     "bta_potrs": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P, _I, _I, _I, _P]),
     "bta_potri": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P]),
-    "bta_assemble_kron": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _P, _P, _I, _P, _D, _P]),
+    "bta_assemble_kron": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _P, _P, _I, _P, _D, _P]),
     "bta_scatter_coo": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _D, _I, _P]),
     "bta_gather_coo": (_I, [_P, _P, _P, _P, _P, _P, C.c_int64, _P]),
     "bta_bilinear_coo": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int64, _I, _P, _I, _P, _P]),
-    "bta_kron_contract": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P]),
+    "bta_kron_contract": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P]),
 }
 
 BTA_E = {-1: "invalid argument", -2: "shape does not match the workspace", -3: "in-kernel wait timed out"}

@@ -117,3 +117,69 @@ def gaussian_log_prior(theta, mean, prec):
 def gaussian_log_prior_grad(theta, mean, prec):
     th = np.asarray(theta, dtype=np.float64)
     return -prec * (th - mean)
+
+
+# ------------------------------------------------------------ coregional
+
+def coregional_weights(lam: torch.Tensor, nv: int) -> torch.Tensor:
+    """C[v][w][u]: Q_vw = sum_u C[v][w][u] Qu_u for the coregional precision.
+
+    Restates CoregionalModel.construct_Q_prior (coregional_model.py:300-408)
+    with the marginal scales sigma_u set to 1 (each field's own theta_e scales
+    its Qu_u):
+      nv = 2: q11 = Qu0 + l01^2 Qu1, q21 = -l01 Qu1, q22 = Qu1
+      nv = 3: q11 = Qu0 + l01^2 Qu1 + l12^2 Qu2, q21 = -l01 Qu1 + l02 l12 Qu2,
+              q31 = -l12 Qu2, q22 = Qu1 + l02^2 Qu2, q32 = -l02 Qu2, q33 = Qu2
+    with lam = (l01,) or (l01, l02, l12); q_vw = q_wv.
+    """
+    z = torch.zeros((), dtype=torch.float64)
+    one = torch.ones((), dtype=torch.float64)
+    if nv == 2:
+        l01 = lam[0]
+        q00 = torch.stack([one, l01 ** 2])
+        q10 = torch.stack([z, -l01])
+        q11 = torch.stack([z, one])
+        return torch.stack([torch.stack([q00, q10]), torch.stack([q10, q11])])
+    if nv == 3:
+        l01, l02, l12 = lam[0], lam[1], lam[2]
+        q00 = torch.stack([one, l01 ** 2, l12 ** 2])
+        q10 = torch.stack([z, -l01, l02 * l12])
+        q20 = torch.stack([z, z, -l12])
+        q11 = torch.stack([z, one, l02 ** 2])
+        q21 = torch.stack([z, z, -l02])
+        q22 = torch.stack([z, z, one])
+        return torch.stack([torch.stack([q00, q10, q20]),
+                            torch.stack([q10, q11, q21]),
+                            torch.stack([q20, q21, q22])])
+    raise ValueError("coregional models have 2 or 3 fields")
+
+
+class CoregionalCoefficients:
+    """Table (nv, nv, nM, nK) of the coregional precision.
+
+    theta = (theta_s, theta_t, theta_e) per field, then the nv(nv-1)/2
+    coregionalisation weights (l01[, l02, l12]), then one noise precision per
+    field.  Field u's own precision uses the benchmark parameterization
+    (baseline_table).
+    """
+
+    def __init__(self, nv: int):
+        self.nv = nv
+        self.nM, self.nK = 3, 3
+        self.n_lam = nv * (nv - 1) // 2
+        self.d_q = 3 * nv + self.n_lam
+
+    def _fn(self, th: torch.Tensor) -> torch.Tensor:
+        nv = self.nv
+        tabs = torch.stack([baseline_table(th[3 * u:3 * u + 3]) for u in range(nv)])  # (u, m, k)
+        C = coregional_weights(th[3 * nv:3 * nv + self.n_lam], nv)  # (v, w, u)
+        return torch.einsum("vwu,umk->vwmk", C, tabs)
+
+    def table(self, theta) -> np.ndarray:
+        th = torch.as_tensor(np.asarray(theta[: self.d_q], dtype=np.float64))
+        return self._fn(th).detach().numpy().copy()
+
+    def jacobian(self, theta) -> np.ndarray:
+        th = torch.as_tensor(np.asarray(theta[: self.d_q], dtype=np.float64))
+        J = torch.autograd.functional.jacobian(self._fn, th)  # (v, w, m, k, d)
+        return J.permute(4, 0, 1, 2, 3).detach().numpy().copy()

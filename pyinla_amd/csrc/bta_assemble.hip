@@ -13,7 +13,7 @@
 
 namespace bta {
 
-#define BTA_MAX_KM 16  // nM * nK coefficient table entries
+#define BTA_MAX_KM 128  // nv * nv * nM * nK coefficient table entries
 
 struct KronArgs {
   double* D;
@@ -25,50 +25,60 @@ struct KronArgs {
   const double* vals;   // nK x nnz
   const double* Mdiag;  // nM x n
   const double* Moff;   // nM x (n-1)
-  double coef[BTA_MAX_KM];  // nM x nK
+  double coef[BTA_MAX_KM];  // [v][w][m][k], nv x nv x nM x nK
   double tip_diag;
-  int n, b, a, nnz, nM, nK;
+  int n, b, a, nnz, nM, nK, nv, ns;  // b = nv * ns
 };
 
 // grid: n * rows_chunks workgroups; each handles 4 rows of block t (one per wave).
+// Row r of block t belongs to field v = r / ns, mesh node s = r % ns; its
+// entries in field w sit at columns w*ns + colidx[e] with value
+//   sum_k (sum_m coef[v][w][m][k] M_m[t][t]) vals[k][e]   (diagonal block)
+//   sum_k (sum_m coef[v][w][m][k] M_m[t+1][t]) vals[k][e] (sub-diagonal block).
 __global__ void __launch_bounds__(BTA_THREADS) k_assemble_kron(KronArgs p) {
   const int chunks = (p.b + 3) / 4;
   const int t = blockIdx.x / chunks;
   const int r = (blockIdx.x % chunks) * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  double wD[4], wL[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    wD[k] = 0.0;
-    wL[k] = 0.0;
-    if (k < p.nK)
-      for (int m = 0; m < p.nM; ++m) {
-        wD[k] = fma(p.coef[m * p.nK + k], p.Mdiag[m * p.n + t], wD[k]);
-        if (t < p.n - 1) wL[k] = fma(p.coef[m * p.nK + k], p.Moff[m * (p.n - 1) + t], wL[k]);
-      }
-  }
   if (r < p.b) {
+    const int v = r / p.ns, srow = r % p.ns;
     double* drow = p.D + ((int64_t)t * p.b + r) * p.b;
     double* lrow = (t < p.n - 1) ? p.L + ((int64_t)t * p.b + r) * p.b : nullptr;
     for (int c = lane; c < p.b; c += 64) {
       drow[c] = 0.0;
       if (lrow) lrow[c] = 0.0;
     }
-    // wave-local: the zero stores above and the value stores below come from
-    // the same wave, so program order keeps the values last.
-    __builtin_amdgcn_wave_barrier();
+    // the zero stores above and the value stores below come from the same
+    // wave; the fence keeps the values last
     __threadfence_block();
-    const int e0 = p.rowptr[r], e1 = p.rowptr[r + 1];
-    for (int e = e0 + lane; e < e1; e += 64) {
-      const int c = p.colidx[e];
-      double vd = 0.0, vl = 0.0;
-      for (int k = 0; k < p.nK && k < 4; ++k) {
-        const double v = p.vals[(int64_t)k * p.nnz + e];
-        vd = fma(wD[k], v, vd);
-        vl = fma(wL[k], v, vl);
+    const int e0 = p.rowptr[srow], e1 = p.rowptr[srow + 1];
+    for (int w = 0; w < p.nv; ++w) {
+      double wD[4], wL[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        wD[k] = 0.0;
+        wL[k] = 0.0;
+        if (k < p.nK)
+          for (int m = 0; m < p.nM; ++m) {
+            const double cf = p.coef[((v * p.nv + w) * p.nM + m) * p.nK + k];
+            wD[k] = fma(cf, p.Mdiag[m * p.n + t], wD[k]);
+            if (t < p.n - 1) wL[k] = fma(cf, p.Moff[m * (p.n - 1) + t], wL[k]);
+          }
       }
-      drow[c] = vd;
-      if (lrow) lrow[c] = vl;
+      for (int e = e0 + lane; e < e1; e += 64) {
+        const int c = w * p.ns + p.colidx[e];
+        double vd = 0.0, vl = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (k < p.nK) {
+            const double val = p.vals[(int64_t)k * p.nnz + e];
+            vd = fma(wD[k], val, vd);
+            vl = fma(wL[k], val, vl);
+          }
+        }
+        drow[c] = vd;
+        if (lrow) lrow[c] = vl;
+      }
     }
   }
   // arrow rows of block t and the tip (block 0 handles the tip)
@@ -148,57 +158,85 @@ __global__ void __launch_bounds__(BTA_THREADS) k_bilinear_partial(const double* 
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// Kronecker contractions, per block t and spatial matrix k:
-//   mode 0 (traces):     PD[t][k] = <D_t, S_k>,  PL[t][k] = <L_t, S_k>
-//   mode 1 (quadratic):  PD[t][k] = x_t^T S_k x_t,  PL[t][k] = x_{t+1}^T S_k x_t
-// grid: n * chunks workgroups (rows in chunks of 64); partials per (t, chunk),
-// reduced afterwards in chunk order.  out layout: part[(t*chunks + ch)*2*nK + ...]
+// Kronecker contractions, per block t, field pair (v, w) and spatial matrix k:
+//   mode 0 (traces):     PD = <D_t[v,w], S_k>_F,  PL = <L_t[v,w], S_k>_F
+//   mode 1 (quadratic):  PD = x_{t,v}^T S_k x_{t,w},  PL = x_{t+1,v}^T S_k x_{t,w}
+// where X[v,w] is the (v, w) ns x ns sub-block of a b x b block (b = nv ns).
+// grid: n * nv * chunks workgroups (64 mesh rows of one field each); partials
+// per (t, v, chunk) are reduced afterwards in chunk order.
+// part layout: [((t*nv + v)*chunks + ch) * width + (w*nK + k)*2 + {0: D, 1: L}],
+// width = nv * nK * 2.
 __global__ void __launch_bounds__(BTA_THREADS) k_kron_contract(const double* D, const double* L,
                                                                const double* x, const int* rowptr,
                                                                const int* colidx, const double* vals, int n,
-                                                               int b, int nnz, int nK, int mode, double* part) {
-  __shared__ double red[4][2 * 4];
-  const int chunks = (b + 63) / 64;
-  const int t = blockIdx.x / chunks, ch = blockIdx.x % chunks;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double accD[4] = {0, 0, 0, 0}, accL[4] = {0, 0, 0, 0};
-  for (int rr = w; rr < 64; rr += 4) {
-    const int r = ch * 64 + rr;
-    if (r >= b) break;
-    const int e0 = rowptr[r], e1 = rowptr[r + 1];
-    for (int e = e0 + lane; e < e1; e += 64) {
-      const int c = colidx[e];
-      double gD, gL = 0.0;
-      if (mode == 0) {
-        gD = D[((int64_t)t * b + r) * b + c];
-        if (t < n - 1) gL = L[((int64_t)t * b + r) * b + c];
-      } else {
-        gD = x[(int64_t)t * b + r] * x[(int64_t)t * b + c];
-        if (t < n - 1) gL = x[(int64_t)(t + 1) * b + r] * x[(int64_t)t * b + c];
-      }
-      for (int k = 0; k < nK && k < 4; ++k) {
-        const double v = vals[(int64_t)k * nnz + e];
-        accD[k] = fma(v, gD, accD[k]);
-        accL[k] = fma(v, gL, accL[k]);
-      }
-    }
-  }
-  for (int k = 0; k < 4; ++k) {
-    double d = accD[k], l = accL[k];
+                                                               int nv, int ns, int nnz, int nK, int mode,
+                                                               double* part) {
+  __shared__ double red[4][32];
+  const int b = nv * ns;
+  const int chunks = (ns + 63) / 64;
+  const int t = blockIdx.x / (nv * chunks);
+  const int v = (blockIdx.x / chunks) % nv;
+  const int ch = blockIdx.x % chunks;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double accD[4][4], accL[4][4];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      d += __shfl_xor(d, off, 64);
-      l += __shfl_xor(l, off, 64);
+  for (int w = 0; w < 4; ++w)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      accD[w][k] = 0.0;
+      accL[w][k] = 0.0;
     }
-    if (lane == 0) {
-      red[w][2 * k] = d;
-      red[w][2 * k + 1] = l;
+  for (int rr = wv; rr < 64; rr += 4) {
+    const int s = ch * 64 + rr;
+    if (s >= ns) break;
+    const int r = v * ns + s;
+    const int e0 = rowptr[s], e1 = rowptr[s + 1];
+    for (int e = e0 + lane; e < e1; e += 64) {
+      const int cs = colidx[e];
+      double vk[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) vk[k] = (k < nK) ? vals[(int64_t)k * nnz + e] : 0.0;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        if (w < nv) {
+          const int c = w * ns + cs;
+          double gD, gL = 0.0;
+          if (mode == 0) {
+            gD = D[((int64_t)t * b + r) * b + c];
+            if (t < n - 1) gL = L[((int64_t)t * b + r) * b + c];
+          } else {
+            gD = x[(int64_t)t * b + r] * x[(int64_t)t * b + c];
+            if (t < n - 1) gL = x[(int64_t)(t + 1) * b + r] * x[(int64_t)t * b + c];
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            accD[w][k] = fma(vk[k], gD, accD[w][k]);
+            accL[w][k] = fma(vk[k], gL, accL[w][k]);
+          }
+        }
+      }
     }
   }
+  const int width = nv * nK * 2;
+#pragma unroll
+  for (int w = 0; w < 4; ++w)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double d = accD[w][k], l = accL[w][k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        d += __shfl_xor(d, off, 64);
+        l += __shfl_xor(l, off, 64);
+      }
+      if (lane == 0 && w < nv && k < nK) {
+        red[wv][(w * nK + k) * 2] = d;
+        red[wv][(w * nK + k) * 2 + 1] = l;
+      }
+    }
   __syncthreads();
-  if (threadIdx.x < 2 * nK) {
+  if ((int)threadIdx.x < width) {
     const int q = threadIdx.x;
-    part[(int64_t)blockIdx.x * 2 * nK + q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
+    part[((int64_t)(t * nv + v) * chunks + ch) * width + q] = (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
   }
 }
 
@@ -233,11 +271,13 @@ __global__ void k_reduce_all(const double* part, int npart, double* out) {
 
 extern "C" {
 
-int bta_assemble_kron(double* D, double* L, double* B, double* T, int n, int b, int a, const int* rowptr,
-                      const int* colidx, const double* vals, int nnz, int nK, const double* Mdiag,
-                      const double* Moff, int nM, const double* coef, double tip_diag, void* stream) {
+int bta_assemble_kron(double* D, double* L, double* B, double* T, int n, int b, int a, int nv,
+                      const int* rowptr, const int* colidx, const double* vals, int nnz, int nK,
+                      const double* Mdiag, const double* Moff, int nM, const double* coef, double tip_diag,
+                      void* stream) {
   if (!D || (n > 1 && !L) || !rowptr || !colidx || !vals || !Mdiag || (n > 1 && !Moff) || !coef) return BTA_E_ARG;
-  if (nK < 1 || nK > 4 || nM < 1 || nM * nK > BTA_MAX_KM || n < 1 || b < 1 || a < 0) return BTA_E_ARG;
+  if (nv < 1 || nv > 4 || b % nv != 0) return BTA_E_ARG;
+  if (nK < 1 || nK > 4 || nM < 1 || nv * nv * nM * nK > BTA_MAX_KM || n < 1 || b < 1 || a < 0) return BTA_E_ARG;
   bta::KronArgs p;
   p.D = D;
   p.L = n > 1 ? L : nullptr;
@@ -248,7 +288,7 @@ int bta_assemble_kron(double* D, double* L, double* B, double* T, int n, int b, 
   p.vals = vals;
   p.Mdiag = Mdiag;
   p.Moff = Moff;
-  for (int q = 0; q < BTA_MAX_KM; ++q) p.coef[q] = q < nM * nK ? coef[q] : 0.0;
+  for (int q = 0; q < BTA_MAX_KM; ++q) p.coef[q] = q < nv * nv * nM * nK ? coef[q] : 0.0;
   p.tip_diag = tip_diag;
   p.n = n;
   p.b = b;
@@ -256,6 +296,8 @@ int bta_assemble_kron(double* D, double* L, double* B, double* T, int n, int b, 
   p.nnz = nnz;
   p.nM = nM;
   p.nK = nK;
+  p.nv = nv;
+  p.ns = b / nv;
   const int chunks = (b + 3) / 4;
   hipLaunchKernelGGL(bta::k_assemble_kron, dim3(n * chunks), dim3(BTA_THREADS), 0,
                      reinterpret_cast<hipStream_t>(stream), p);
@@ -295,18 +337,20 @@ int bta_bilinear_coo(const double* D, const double* L, const double* B, const do
   return (int)hipGetLastError();
 }
 
-int bta_kron_contract(const double* D, const double* L, const double* x, int n, int b, const int* rowptr,
+int bta_kron_contract(const double* D, const double* L, const double* x, int n, int b, int nv, const int* rowptr,
                       const int* colidx, const double* vals, int nnz, int nK, int mode, double* work,
                       double* out, void* stream) {
-  if (nK < 1 || nK > 4 || !rowptr || !colidx || !vals || !work || !out) return BTA_E_ARG;
+  if (nK < 1 || nK > 4 || nv < 1 || nv > 4 || b % nv != 0 || !rowptr || !colidx || !vals || !work || !out)
+    return BTA_E_ARG;
   if ((mode == 0 && !D) || (mode == 1 && !x)) return BTA_E_ARG;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const int chunks = (b + 63) / 64;
-  hipLaunchKernelGGL(bta::k_kron_contract, dim3(n * chunks), dim3(BTA_THREADS), 0, st, D, L, x, rowptr, colidx,
-                     vals, n, b, nnz, nK, mode, work);
-  const int width = 2 * nK;
-  hipLaunchKernelGGL(bta::k_reduce_chunks, dim3((n * width + 255) / 256), dim3(256), 0, st, work, n, chunks, width,
-                     out);
+  const int ns = b / nv;
+  const int chunks = (ns + 63) / 64;
+  hipLaunchKernelGGL(bta::k_kron_contract, dim3(n * nv * chunks), dim3(BTA_THREADS), 0, st, D, L, x, rowptr,
+                     colidx, vals, n, nv, ns, nnz, nK, mode, work);
+  const int width = nv * nK * 2;
+  hipLaunchKernelGGL(bta::k_reduce_chunks, dim3((n * nv * width + 255) / 256), dim3(256), 0, st, work, n * nv,
+                     chunks, width, out);
   return (int)hipGetLastError();
 }
 

@@ -228,6 +228,110 @@ int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
   return rc;
 }
 
+}  // extern "C"
+
+struct bta_batch {
+  bta::BatchPlan plan;
+  std::vector<bta_ws*> ws;
+  int device = 0;
+  int use_graph = 0;
+  hipStream_t gstream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B, double** T, const int* a,
+                     int use_graph, bta_batch_t* out) {
+  if (!out || nprob < 1 || !ws || !D || !L || !a) return BTA_E_ARG;
+  *out = nullptr;
+  std::vector<bta::ProbDesc> ps;
+  for (int q = 0; q < nprob; ++q) {
+    bta_ws* w = ws[q];
+    if (!w || !D[q]) return BTA_E_ARG;
+    if (a[q] != 0 && a[q] != w->w.a) return BTA_E_SHAPE;
+    if (w->w.n > 1 && !L[q]) return BTA_E_ARG;
+    if (a[q] > 0 && (!B || !T || !B[q] || !T[q])) return BTA_E_ARG;
+    if (w->w.device != ws[0]->w.device) return BTA_E_ARG;
+    bta::ProbDesc d;
+    d.D = D[q];
+    d.L = L[q];
+    d.B = a[q] > 0 ? B[q] : nullptr;
+    d.T = a[q] > 0 ? T[q] : nullptr;
+    d.winv = w->w.winv;
+    d.logpart = w->w.logpart;
+    d.flags = w->w.flags;
+    d.epoch = w->w.epoch;
+    d.info = w->w.info;
+    d.err = w->w.err;
+    d.logdet = w->w.logdet;
+    d.tinv = w->w.tinv;
+    d.spin_limit = w->w.spin_limit;
+    d.n = w->w.n;
+    d.b = w->w.b;
+    d.a = a[q];
+    d.nT = w->w.nT;
+    d.nTa = (a[q] + 63) / 64;
+    ps.push_back(d);
+  }
+  hipError_t e = hipSetDevice(ws[0]->w.device);
+  if (e != hipSuccess) return (int)e;
+  bta_batch* bt = new bta_batch();
+  bt->device = ws[0]->w.device;
+  bt->ws.assign(ws, ws + nprob);
+  const int rc = bt->plan.build(ps);
+  if (rc) {
+    delete bt;
+    return rc;
+  }
+  bt->use_graph = use_graph;
+  if (use_graph) {
+    hipStreamCreateWithFlags(&bt->gstream, hipStreamNonBlocking);
+    hipEventCreateWithFlags(&bt->ev_in, hipEventDisableTiming);
+    hipEventCreateWithFlags(&bt->ev_out, hipEventDisableTiming);
+  }
+  *out = bt;
+  return 0;
+}
+
+int bta_batch_potrf(bta_batch_t bt, void* stream) {
+  if (!bt) return BTA_E_ARG;
+  for (bta_ws* w : bt->ws) w->w.w_valid = false;
+  hipStream_t st = as_stream(stream);
+  if (!bt->use_graph) return bt->plan.enqueue(st);
+  hipError_t e;
+  if (!bt->exec) {
+    hipGraph_t graph;
+    if ((e = hipStreamBeginCapture(bt->gstream, hipStreamCaptureModeThreadLocal)) != hipSuccess) return (int)e;
+    const int rc = bt->plan.enqueue(bt->gstream);
+    e = hipStreamEndCapture(bt->gstream, &graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return (int)e;
+    e = hipGraphInstantiate(&bt->exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return (int)e;
+  }
+  if ((e = hipEventRecord(bt->ev_in, st)) != hipSuccess) return (int)e;
+  if ((e = hipStreamWaitEvent(bt->gstream, bt->ev_in, 0)) != hipSuccess) return (int)e;
+  if ((e = hipGraphLaunch(bt->exec, bt->gstream)) != hipSuccess) return (int)e;
+  if ((e = hipEventRecord(bt->ev_out, bt->gstream)) != hipSuccess) return (int)e;
+  if ((e = hipStreamWaitEvent(st, bt->ev_out, 0)) != hipSuccess) return (int)e;
+  return 0;
+}
+
+int bta_batch_destroy(bta_batch_t bt) {
+  if (!bt) return 0;
+  hipSetDevice(bt->device);
+  hipDeviceSynchronize();
+  if (bt->exec) hipGraphExecDestroy(bt->exec);
+  if (bt->gstream) hipStreamDestroy(bt->gstream);
+  if (bt->ev_in) hipEventDestroy(bt->ev_in);
+  if (bt->ev_out) hipEventDestroy(bt->ev_out);
+  delete bt;
+  return 0;
+}
+
 int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* stream) {
   if (!ws) return BTA_E_ARG;
   hipStream_t st = as_stream(stream);

@@ -73,6 +73,40 @@ struct TimedScope {
   }
 };
 
+// One problem of a batched factorization (pointers into its own buffers and
+// workspace).
+struct ProbDesc {
+  double *D, *L, *B, *T;
+  double* winv;
+  double* logpart;
+  unsigned long long* flags;
+  unsigned long long* epoch;
+  unsigned long long* info;
+  int* err;
+  double* logdet;
+  double* tinv;
+  unsigned long long spin_limit;
+  int n, b, a, nT, nTa;
+};
+
+// Launch sequence of a batched factorization: for every block step i and
+// tile column k, one panel launch over all problems that have it, and after
+// the last column of step i one grouped update launch.
+struct BatchPlan {
+  struct Launch { int i, k, nent, total, off_prob, off_begin, update; };
+  std::vector<ProbDesc> probs;
+  std::vector<Launch> launches;
+  ProbDesc* d_probs = nullptr;
+  int* d_ent = nullptr;
+  DevPlan updates;
+  BatchPlan() = default;
+  BatchPlan(const BatchPlan&) = delete;
+  BatchPlan& operator=(const BatchPlan&) = delete;
+  ~BatchPlan();
+  int build(const std::vector<ProbDesc>& ps);
+  int enqueue(hipStream_t st) const;
+};
+
 int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
                   hipStream_t st);
 // Host-side preparation (allocations, job tables); never called during capture.

@@ -96,11 +96,9 @@ __host__ __device__ inline int panel_grid(int nT, int nTL, int nTa, int k) {
   return n;
 }
 
-__global__ void __launch_bounds__(BTA_THREADS) k_panel_step(PanelArgs p) {
-  __shared__ double sA[64 * BTA_LDS_LD];
-  __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ double sScr[4 * 64 + 64];
-  const TileRef t = decode_panel_tile(p, blockIdx.x);
+__device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* __restrict__ sA,
+                                           double* __restrict__ sB, double* __restrict__ sScr) {
+  const TileRef t = decode_panel_tile(p, idx);
   const unsigned long long ep = *p.epoch;
   const int b = p.b, k = p.k;
 
@@ -180,6 +178,67 @@ __global__ void __launch_bounds__(BTA_THREADS) k_panel_step(PanelArgs p) {
   const int kend = ((threadIdx.x >> 6) & 1) * 32 + 32;
   mma_nt<false>(x, sA, sB, 0, kend);
   acc_store(x, ct, b, rows, cols);
+}
+
+__global__ void __launch_bounds__(BTA_THREADS) k_panel_step(PanelArgs p) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  __shared__ double sScr[4 * 64 + 64];
+  panel_body(p, blockIdx.x, sA, sB, sScr);
+}
+
+// Batched form: one launch covers tile column k of block step i of every
+// problem in the batch that has such a step.  ent_begin[e] is the first flat
+// workgroup of entry e (ent_begin[nent] = total), ent_prob[e] its problem.
+// Each problem's diagonal-tile workgroup precedes its consumers in the flat
+// order, so the in-order dispatch argument of k_panel_step still holds.
+__device__ __forceinline__ PanelArgs panel_args_of(const ProbDesc& d, int i, int k) {
+  PanelArgs p;
+  const int b = d.b, a = d.a, n = d.n;
+  p.D = d.D + (int64_t)i * b * b;
+  p.L = (i < n - 1) ? d.L + (int64_t)i * b * b : nullptr;
+  p.B = (a > 0) ? d.B + (int64_t)i * a * b : nullptr;
+  p.Winv = d.winv + (int64_t)i * d.nT * 4096;
+  p.logpart = d.logpart + (int64_t)i * d.nT;
+  p.flags = d.flags + (int64_t)i * d.nT;
+  p.epoch = d.epoch;
+  p.info = d.info;
+  p.err = d.err;
+  p.row0 = (long long)i * b;
+  p.spin_limit = d.spin_limit;
+  p.b = b;
+  p.a = a;
+  p.nT = d.nT;
+  p.nTL = p.L ? d.nT : 0;
+  p.nTa = p.B ? d.nTa : 0;
+  p.k = k;
+  return p;
+}
+
+__global__ void __launch_bounds__(BTA_THREADS) k_panel_step_batched(const ProbDesc* __restrict__ probs,
+                                                                   const int* __restrict__ ent_prob,
+                                                                   const int* __restrict__ ent_begin, int nent,
+                                                                   int i, int k) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  __shared__ double sScr[4 * 64 + 64];
+  int lo = 0, hi = nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ent_begin[mid] <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const PanelArgs p = panel_args_of(probs[ent_prob[lo]], i, k);
+  panel_body(p, blockIdx.x - ent_begin[lo], sA, sB, sScr);
+}
+
+__global__ void k_begin_batch(const ProbDesc* probs, int nprob) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nprob) {
+    probs[q].epoch[0] = probs[q].epoch[0] + 1ull;
+    probs[q].info[0] = kInfoNone;
+    probs[q].err[0] = 0;
+  }
 }
 
 // ------------------------------------------------------- block update GEMMs
@@ -283,6 +342,42 @@ __global__ void __launch_bounds__(BTA_THREADS) k_logdet_reduce(const double* par
   if (threadIdx.x == 0) out[0] = 2.0 * s[0];
 }
 
+__global__ void __launch_bounds__(BTA_THREADS) k_tip_potrf_batched(const ProbDesc* probs) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  __shared__ double sScr[4 * 64 + 64];
+  const ProbDesc& d = probs[blockIdx.x];
+  const int a = d.a;
+  if (a <= 0) return;
+  lds_load_tile(sA, d.T, a, a, a);
+  __syncthreads();
+  const TileCholResult res = tile_chol_inv(sA, sB, sScr, a);
+  const int c = threadIdx.x & 63;
+  for (int r = threadIdx.x >> 6; r < 64; r += 4) {
+    if (r < a && c < a) d.T[(int64_t)r * a + c] = sA[r * BTA_LDS_LD + c];
+    d.tinv[r * 64 + c] = sB[r * BTA_LDS_LD + c];
+  }
+  if (threadIdx.x == 0) {
+    d.logpart[(int64_t)d.n * d.nT] = res.logsum;
+    if (res.fail >= 0) atomicMin(d.info, (unsigned long long)((long long)d.n * d.b + res.fail));
+  }
+}
+
+__global__ void __launch_bounds__(BTA_THREADS) k_logdet_reduce_batched(const ProbDesc* probs) {
+  __shared__ double s[BTA_THREADS];
+  const ProbDesc& d = probs[blockIdx.x];
+  const int npart = d.n * d.nT + (d.a > 0 ? 1 : 0);
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < npart; i += BTA_THREADS) acc += d.logpart[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = BTA_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) d.logdet[0] = 2.0 * s[0];
+}
+
 // ------------------------------------------------------------ layout check
 
 __global__ void k_selftest_mfma(double* out) {
@@ -362,6 +457,97 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     npart += 1;
   }
   hipLaunchKernelGGL(k_logdet_reduce, dim3(1), dim3(BTA_THREADS), 0, st, ws.logpart, npart, ws.logdet);
+  return (int)hipGetLastError();
+}
+
+BatchPlan::~BatchPlan() {
+  if (d_probs) hipFree(d_probs);
+  if (d_ent) hipFree(d_ent);
+}
+
+int BatchPlan::build(const std::vector<ProbDesc>& ps) {
+  probs = ps;
+  int nmax = 0;
+  for (const auto& d : ps) nmax = std::max(nmax, d.n);
+  std::vector<int> ent;  // per launch: [prob..., begin..., total]
+  launches.clear();
+  std::vector<GemmLaunch> upd;
+  for (int i = 0; i < nmax; ++i) {
+    int ntmax = 0;
+    for (const auto& d : ps)
+      if (i < d.n) ntmax = std::max(ntmax, d.nT);
+    for (int k = 0; k < ntmax; ++k) {
+      std::vector<int> pr, be;
+      int total = 0;
+      for (int q = 0; q < (int)ps.size(); ++q) {
+        const ProbDesc& d = ps[q];
+        if (i >= d.n || k >= d.nT) continue;
+        const int nTL = (i < d.n - 1) ? d.nT : 0;
+        const int nTa = d.a > 0 ? d.nTa : 0;
+        pr.push_back(q);
+        be.push_back(total);
+        total += panel_grid(d.nT, nTL, nTa, k);
+      }
+      Launch L;
+      L.i = i;
+      L.k = k;
+      L.nent = (int)pr.size();
+      L.total = total;
+      L.off_prob = (int)ent.size();
+      ent.insert(ent.end(), pr.begin(), pr.end());
+      L.off_begin = (int)ent.size();
+      ent.insert(ent.end(), be.begin(), be.end());
+      ent.push_back(total);
+      L.update = -1;
+      launches.push_back(L);
+    }
+    GemmLaunch g;
+    for (const auto& d : ps) {
+      if (i >= d.n) continue;
+      const int b = d.b, a = d.a, n = d.n;
+      double* Li = d.L + (int64_t)i * b * b;
+      double* Bi = a > 0 ? d.B + (int64_t)i * a * b : nullptr;
+      if (i < n - 1) {
+        GemmJob j = job(d.D + (int64_t)(i + 1) * b * b, b, b, b, 1.0, 1, 0);
+        add_seg(j, seg(Li, b, 0, Li, b, 1, b, -1.0));
+        g.add(j);
+        if (a > 0) {
+          GemmJob jb = job(d.B + (int64_t)(i + 1) * a * b, a, b, b, 1.0);
+          add_seg(jb, seg(Bi, b, 0, Li, b, 1, b, -1.0));
+          g.add(jb);
+        }
+      }
+      if (a > 0) {
+        GemmJob jt = job(d.T, a, a, a, 1.0, 1, 0);
+        add_seg(jt, seg(Bi, b, 0, Bi, b, 1, b, -1.0));
+        g.add(jt);
+      }
+    }
+    launches.back().update = (int)upd.size();
+    upd.push_back(std::move(g));
+  }
+  hipError_t e;
+  if ((e = hipMalloc(&d_probs, ps.size() * sizeof(ProbDesc))) != hipSuccess) return (int)e;
+  if ((e = hipMemcpy(d_probs, ps.data(), ps.size() * sizeof(ProbDesc), hipMemcpyHostToDevice)) != hipSuccess)
+    return (int)e;
+  if ((e = hipMalloc(&d_ent, std::max<size_t>(1, ent.size()) * sizeof(int))) != hipSuccess) return (int)e;
+  if (!ent.empty() &&
+      (e = hipMemcpy(d_ent, ent.data(), ent.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return (int)e;
+  return updates.build(upd);
+}
+
+int BatchPlan::enqueue(hipStream_t st) const {
+  const int np = (int)probs.size();
+  hipLaunchKernelGGL(k_begin_batch, dim3((np + 63) / 64), dim3(64), 0, st, d_probs, np);
+  for (const Launch& L : launches) {
+    if (L.total > 0)
+      hipLaunchKernelGGL(k_panel_step_batched, dim3(L.total), dim3(BTA_THREADS), 0, st, d_probs,
+                         d_ent + L.off_prob, d_ent + L.off_begin, L.nent, L.i, L.k);
+    if (L.update >= 0) updates.launch(L.update, st);
+  }
+  hipLaunchKernelGGL(k_tip_potrf_batched, dim3(np), dim3(BTA_THREADS), 0, st, d_probs);
+  hipLaunchKernelGGL(k_logdet_reduce_batched, dim3(np), dim3(BTA_THREADS), 0, st, d_probs);
   return (int)hipGetLastError();
 }
 

@@ -76,25 +76,58 @@ def bta_codes(rows: np.ndarray, cols: np.ndarray, n: int, b: int, a: int) -> np.
     return code
 
 
-class STGaussianModel:
-    """Gaussian spatio-temporal model with its two BTA solvers resident on the GPU.
+class _ObsSet:
+    """One observation block (A_v, y_v) with its own noise precision theta[idx]."""
 
-    Holds Q_cond (BTA: b = ns, n = nt, a = nb) and Q_prior's space-time part
-    (BT) in separate solvers so that both factors stay available for the
-    adjoint.  ``coef`` selects the coefficient table (see coefficients.py).
+    def __init__(self, A, y, idx, n, b, a, dev):
+        A = sps.csr_matrix(A)
+        AtA = sps.tril(sps.csr_matrix(A.T @ A)).tocoo()
+        AtA.sum_duplicates()
+        self.idx = idx
+        self.code = torch.from_numpy(bta_codes(AtA.row, AtA.col, n, b, a).view(np.int64)).to(dev)
+        self.val = torch.from_numpy(AtA.data.copy()).to(dev)
+        w = np.where(AtA.row == AtA.col, 1.0, 2.0)
+        self.wval = torch.from_numpy(AtA.data * w).to(dev)
+        self.row = torch.from_numpy(AtA.row.astype(np.int32)).to(dev)
+        self.col = torch.from_numpy(AtA.col.astype(np.int32)).to(dev)
+        self.count = int(AtA.nnz)
+        self.Aty = torch.from_numpy(np.asarray(A.T @ y, dtype=np.float64)).to(dev)
+        self.yty = float(np.asarray(y) @ np.asarray(y))
+        self.m = A.shape[0]
+
+
+class STGaussianModel:
+    """Gaussian (spatio-temporal or coregional) model with its two BTA solvers
+    resident on the GPU.
+
+    Holds Q_cond (BTA: n = nt, b = nv * ns, a = total fixed effects) and the
+    space-time part of Q_prior (BT) in separate solvers so that both factors
+    stay available for the adjoint.  ``coef`` maps theta to the coefficient
+    table (see coefficients.py); nv = 1 is the single-field model, nv = 2 or 3
+    the coregional model with fields interleaved per time block
+    (coregional_model.py:754-786) and one noise precision per field.
     """
 
-    def __init__(self, prob, coef: CoefficientModel | None = None, device=None, use_graph: bool = True):
+    def __init__(self, prob, coef=None, device=None, use_graph: bool = True):
+        from .coefficients import CoregionalCoefficients
         self.prob = prob
-        self.coef = coef if coef is not None else CoefficientModel("baseline")
+        nv = getattr(prob, "nv", 1)
+        if coef is None:
+            coef = CoefficientModel("baseline") if nv == 1 else CoregionalCoefficients(nv)
+        self.coef = coef
+        self.nv = nv
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
         self.lib = _lib.load()
-        ns, nt, nb = prob.ns, prob.nt, prob.nb
-        self.n, self.b, self.a = nt, ns, nb
+        ns, nt = prob.ns, prob.nt
+        nb_total = prob.nb
+        self.ns = ns
+        self.n, self.b, self.a = nt, nv * ns, nb_total
         dev = self.device
+        self.nK = coef.nK
+        self.d_q = getattr(coef, "d_q", getattr(coef, "d_st", 3))
         # spatial matrices on one union CSR pattern
-        S = [sps.csr_matrix(s) for s in prob.spatial][: self.coef.nK]
-        while len(S) < self.coef.nK:
+        S = [sps.csr_matrix(s) for s in prob.spatial][: self.nK]
+        while len(S) < self.nK:
             S.append(sps.csr_matrix((ns, ns)))
         union = abs(S[0])
         for s in S[1:]:
@@ -109,8 +142,7 @@ class STGaussianModel:
             pat.data[:] = 0.0
             v = (pat + s2).tocsr()
             v.sort_indices()
-            # same pattern as union (union contains every entry of s2)
-            assert v.nnz == union.nnz
+            assert v.nnz == union.nnz  # union contains every entry of s2
             vals.append(v.data.copy())
         self.nnz = union.nnz
         self.rowptr = torch.from_numpy(union.indptr.astype(np.int32)).to(dev)
@@ -127,36 +159,38 @@ class STGaussianModel:
             Md[m] = M.diagonal()
             if nt > 1:
                 Mo[m, : nt - 1] = M.diagonal(-1)
-        self.Mdiag_np, self.Moff_np = Md, Mo[:, : max(nt - 1, 0)] if nt > 1 else Mo[:, :0]
+        self.Mdiag_np = Md
+        self.Moff_np = Mo[:, : nt - 1] if nt > 1 else Mo[:, :0]
         self.Mdiag = torch.from_numpy(Md).to(dev)
         self.Moff = torch.from_numpy(np.ascontiguousarray(Mo)).to(dev)
         self.nM = nM
-        # A^T A on the lower BTA pattern, with symmetry weights
-        A = sps.csr_matrix(prob.A)
-        AtA = sps.tril(sps.csr_matrix(A.T @ A)).tocoo()
-        AtA.sum_duplicates()
-        self.ata_code = torch.from_numpy(bta_codes(AtA.row, AtA.col, nt, ns, nb).view(np.int64)).to(dev)
-        self.ata_val = torch.from_numpy(AtA.data.copy()).to(dev)
-        w = np.where(AtA.row == AtA.col, 1.0, 2.0)
-        self.ata_wval = torch.from_numpy(AtA.data * w).to(dev)
-        self.ata_row = torch.from_numpy(AtA.row.astype(np.int32)).to(dev)
-        self.ata_col = torch.from_numpy(AtA.col.astype(np.int32)).to(dev)
-        self.ata_count = int(AtA.nnz)
-        self.Aty = torch.from_numpy(np.asarray(A.T @ prob.y, dtype=np.float64)).to(dev)
-        self.yty = float(prob.y @ prob.y)
-        self.m_obs = A.shape[0]
+        # observation blocks, each with its noise precision index in theta
+        n, b, a = self.n, self.b, self.a
+        if getattr(prob, "A_list", None) is not None:
+            idxs = [self.d_q + v for v in range(len(prob.A_list))]
+            self.obs = [_ObsSet(A, y, i, n, b, a, dev) for A, y, i in zip(prob.A_list, prob.y_list, idxs)]
+        else:
+            self.obs = [_ObsSet(prob.A, prob.y, self.d_q, n, b, a, dev)]
+        self.m_obs = sum(o.m for o in self.obs)
         # solvers: conditional (BTA) and prior space-time part (BT)
-        self.cond = SerinvSolver(None, ns, nt, nb, device=dev, use_graph=use_graph)
-        self.prior = SerinvSolver(None, ns, nt, 0, device=dev, use_graph=use_graph)
-        self.x = torch.zeros(nt * ns + nb, dtype=torch.float64, device=dev)
+        self.cond = SerinvSolver(None, b, nt, a, device=dev, use_graph=use_graph)
+        self.prior = SerinvSolver(None, b, nt, 0, device=dev, use_graph=use_graph)
+        self.x = torch.zeros(nt * b + a, dtype=torch.float64, device=dev)
         self.rhs = torch.zeros_like(self.x)
         chunks = (ns + 63) // 64
-        self.work = torch.zeros(max(nt * chunks * 8, 4096), dtype=torch.float64, device=dev)
-        self.contract_out = torch.zeros(nt * 8, dtype=torch.float64, device=dev)
+        self.work = torch.zeros(max(nt * nv * chunks * 2 * nv * self.nK, 4096), dtype=torch.float64, device=dev)
+        self.contract_out = torch.zeros(nt * nv * nv * self.nK * 2, dtype=torch.float64, device=dev)
         self.scalar = torch.zeros(4, dtype=torch.float64, device=dev)
-        self.timers = {}
 
     # ---------------------------------------------------------- assembly
+
+    def _table(self, theta) -> np.ndarray:
+        t = np.asarray(self.coef.table(theta), dtype=np.float64)
+        return t.reshape(self.nv, self.nv, self.nM, self.nK)
+
+    def _jacobian(self, theta) -> np.ndarray:
+        J = np.asarray(self.coef.jacobian(theta), dtype=np.float64)
+        return J.reshape(J.shape[0], self.nv, self.nv, self.nM, self.nK)
 
     def _assemble(self, solver: SerinvSolver, table: np.ndarray, with_arrow: bool) -> None:
         n, b, a = self.n, self.b, self.a
@@ -166,20 +200,34 @@ class STGaussianModel:
         Tp = solver.A_arrow_tip_block if with_arrow else None
         _lib.check(self.lib.bta_assemble_kron(
             _ptr(solver.A_diagonal_blocks), _ptr(solver.A_lower_diagonal_blocks), _ptr(Bp), _ptr(Tp),
-            n, b, arrow_a, _ptr(self.rowptr), _ptr(self.colidx), _ptr(self.vals), self.nnz, self.coef.nK,
+            n, b, arrow_a, self.nv, _ptr(self.rowptr), _ptr(self.colidx), _ptr(self.vals), self.nnz, self.nK,
             _ptr(self.Mdiag), _ptr(self.Moff), self.nM, coef.ctypes.data_as(C.c_void_p),
             float(self.prob.fixed_prec), _stream_ptr(self.device)), "bta_assemble_kron")
 
-    def assemble_prior(self, theta) -> None:
-        self._assemble(self.prior, self.coef.table(theta), with_arrow=False)
+    def assemble_prior(self, theta, solver=None) -> None:
+        self._assemble(solver or self.prior, self._table(theta), with_arrow=False)
 
-    def assemble_cond(self, theta) -> None:
-        s = self.cond
-        self._assemble(s, self.coef.table(theta), with_arrow=True)
-        _lib.check(self.lib.bta_scatter_coo(
-            _ptr(s.A_diagonal_blocks), _ptr(s.A_lower_diagonal_blocks), _ptr(s.A_arrow_bottom_blocks),
-            _ptr(s.A_arrow_tip_block), _ptr(self.ata_code), _ptr(self.ata_val), None, self.ata_count,
-            float(np.exp(theta[3])), 1, _stream_ptr(self.device)), "bta_scatter_coo")
+    def assemble_cond(self, theta, solver=None) -> None:
+        s = solver or self.cond
+        self._assemble(s, self._table(theta), with_arrow=True)
+        for o in self.obs:
+            _lib.check(self.lib.bta_scatter_coo(
+                _ptr(s.A_diagonal_blocks), _ptr(s.A_lower_diagonal_blocks), _ptr(s.A_arrow_bottom_blocks),
+                _ptr(s.A_arrow_tip_block), _ptr(o.code), _ptr(o.val), None, o.count,
+                float(np.exp(theta[o.idx])), 1, _stream_ptr(self.device)), "bta_scatter_coo")
+
+    def _rhs(self, theta) -> None:
+        self.rhs.zero_()
+        for o in self.obs:
+            self.rhs.add_(o.Aty, alpha=float(np.exp(theta[o.idx])))
+
+    def _tail(self, theta, ld_c, ld_p, quad):
+        logp = gaussian_log_prior(theta, self.prob.prior_mean, self.prob.prior_prec)
+        lik = sum(0.5 * theta[o.idx] * o.m - 0.5 * np.exp(theta[o.idx]) * o.yty for o in self.obs)
+        f = 0.5 * ld_c - 0.5 * quad - logp - lik - 0.5 * ld_p
+        if np.isnan(f):
+            raise ValueError("f(theta) is NaN. Check what is happening.")
+        return dict(f=f, ld_c=ld_c, ld_p=ld_p, quad=quad, logp=logp, lik=lik)
 
     # -------------------------------------------------------------- f(theta)
 
@@ -190,8 +238,7 @@ class STGaussianModel:
         self.cond.factorize("bta")
         self.assemble_prior(theta)
         self.prior.factorize("bt")
-        eo = float(np.exp(theta[3]))
-        torch.mul(self.Aty, eo, out=self.rhs)
+        self._rhs(theta)
         self.x.copy_(self.rhs)
         self.cond.solve(self.x, "bta")
         info_c, ld_c = self.cond.status()
@@ -199,54 +246,92 @@ class STGaussianModel:
         if info_c or info_p:
             raise np.linalg.LinAlgError(f"not positive definite (cond info {info_c}, prior info {info_p})")
         quad = float(torch.dot(self.x, self.rhs))
-        logp = gaussian_log_prior(theta, self.prob.prior_mean, self.prob.prior_prec)
-        lik = 0.5 * theta[3] * self.m_obs - 0.5 * eo * self.yty
-        f = 0.5 * ld_c - 0.5 * quad - logp - lik - 0.5 * ld_p
-        if np.isnan(f):
-            raise ValueError("f(theta) is NaN. Check what is happening.")
-        return dict(f=f, ld_c=ld_c, ld_p=ld_p, quad=quad, logp=logp, lik=lik)
+        return self._tail(theta, ld_c, ld_p, quad)
 
     def f(self, theta) -> float:
         return self.forward(theta)["f"]
+
+    @staticmethod
+    def _stencil(theta, eps):
+        d = theta.size
+        mat = np.repeat(theta[:, None], 2 * d + 1, axis=1)
+        mat[:, 1:1 + d] += eps * np.eye(d)
+        mat[:, d + 1:] -= eps * np.eye(d)
+        return mat
 
     def fd_gradient(self, theta, eps: float = 1e-3):
         """Central differences as PyINLA._objective_function (pyinla.py:512-600)."""
         theta = np.asarray(theta, dtype=np.float64)
         d = theta.size
-        mat = np.repeat(theta[:, None], 2 * d + 1, axis=1)
-        mat[:, 1:1 + d] += eps * np.eye(d)
-        mat[:, d + 1:] -= eps * np.eye(d)
+        mat = self._stencil(theta, eps)
         fv = np.zeros(2 * d + 1)
         for k in range(2 * d, -1, -1):
             fv[k] = self.f(mat[:, k])
-        grad = (fv[1:1 + d] - fv[d + 1:]) / (2 * eps)
-        return fv[0], grad
+        return fv[0], (fv[1:1 + d] - fv[d + 1:]) / (2 * eps)
+
+    def fd_gradient_batched(self, theta, eps: float = 1e-3, wave: int | None = None):
+        """The same FD gradient with the factorizations of up to ``wave``
+        stencil points (default all 2d+1) in one batched, graph-replayed launch
+        sequence; one extra solver pair per stencil point of a wave."""
+        from .batch import BatchedFactorization
+        theta = np.asarray(theta, dtype=np.float64)
+        d = theta.size
+        npts = 2 * d + 1
+        wave = npts if wave is None else max(1, min(wave, npts))
+        mat = self._stencil(theta, eps)
+        pool = getattr(self, "_fd_pool", None)
+        if pool is None or len(pool[0]) != wave:
+            n, b, a = self.n, self.b, self.a
+            conds = [self.cond] + [SerinvSolver(None, b, n, a, device=self.device) for _ in range(wave - 1)]
+            priors = [self.prior] + [SerinvSolver(None, b, n, 0, device=self.device) for _ in range(wave - 1)]
+            batch = BatchedFactorization(conds + priors, ["bta"] * wave + ["bt"] * wave)
+            self._fd_pool = (conds, priors, batch)
+        conds, priors, batch = self._fd_pool
+        fv = np.zeros(npts)
+        order = list(range(npts - 1, -1, -1))
+        for w0 in range(0, npts, wave):
+            pts = order[w0:w0 + wave]
+            for j, k in enumerate(pts):
+                self.assemble_cond(mat[:, k], conds[j])
+                self.assemble_prior(mat[:, k], priors[j])
+            batch.factorize()
+            for j, k in enumerate(pts):
+                th = mat[:, k]
+                self._rhs(th)
+                self.x.copy_(self.rhs)
+                conds[j].solve(self.x, "bta")
+                info_c, ld_c = conds[j].status()
+                info_p, ld_p = priors[j].status()
+                if info_c or info_p:
+                    raise np.linalg.LinAlgError("not positive definite in the FD batch")
+                quad = float(torch.dot(self.x, self.rhs))
+                fv[k] = self._tail(th, ld_c, ld_p, quad)["f"]
+        return fv[0], (fv[1:1 + d] - fv[d + 1:]) / (2 * eps)
 
     # ---------------------------------------------------------- adjoint
 
     def _contract(self, D, L, x, mode):
-        n, b = self.n, self.b
-        nK = self.coef.nK
+        """Z[v][w][m][k] = sum_t Mdiag[m][t] PD[t][v][w][k] + 2 sum_t Moff[m][t] PL[t][v][w][k]."""
+        n, b, nv, nK = self.n, self.b, self.nv, self.nK
         _lib.check(self.lib.bta_kron_contract(
-            _ptr(D), _ptr(L), _ptr(x), n, b, _ptr(self.rowptr), _ptr(self.colidx), _ptr(self.vals),
+            _ptr(D), _ptr(L), _ptr(x), n, b, nv, _ptr(self.rowptr), _ptr(self.colidx), _ptr(self.vals),
             self.nnz, nK, mode, _ptr(self.work), _ptr(self.contract_out), _stream_ptr(self.device)),
             "bta_kron_contract")
-        out = self.contract_out[: n * 2 * nK].reshape(n, nK, 2).cpu().numpy()
-        PD, PL = out[:, :, 0], out[:, :, 1]
-        # Z[m][k] = sum_t Mdiag[m][t] PD[t][k] + 2 sum_t Moff[m][t] PL[t][k]
-        Z = self.Mdiag_np @ PD
+        out = self.contract_out[: n * nv * nv * nK * 2].reshape(n, nv, nv, nK, 2).cpu().numpy()
+        PD, PL = out[..., 0], out[..., 1]
+        Z = np.einsum("mt,tvwk->vwmk", self.Mdiag_np, PD)
         if n > 1:
-            Z = Z + 2.0 * (self.Moff_np @ PL[: n - 1])
+            Z = Z + 2.0 * np.einsum("mt,tvwk->vwmk", self.Moff_np, PL[: n - 1])
         return Z
 
-    def _bilinear(self, mode, solver=None, x=None):
+    def _bilinear(self, o: _ObsSet, mode, solver=None, x=None):
         s = solver
         nwork = min(1024, self.work.numel())
         _lib.check(self.lib.bta_bilinear_coo(
             _ptr(s.A_diagonal_blocks) if s else None, _ptr(s.A_lower_diagonal_blocks) if s else None,
             _ptr(s.A_arrow_bottom_blocks) if s else None, _ptr(s.A_arrow_tip_block) if s else None,
-            _ptr(self.ata_code), _ptr(self.ata_row), _ptr(self.ata_col), _ptr(x), _ptr(self.ata_wval),
-            self.ata_count, mode, _ptr(self.work), nwork, _ptr(self.scalar), _stream_ptr(self.device)),
+            _ptr(o.code), _ptr(o.row), _ptr(o.col), _ptr(x), _ptr(o.wval),
+            o.count, mode, _ptr(self.work), nwork, _ptr(self.scalar), _stream_ptr(self.device)),
             "bta_bilinear_coo")
         return float(self.scalar[0])
 
@@ -254,21 +339,23 @@ class STGaussianModel:
         """f(theta) and its adjoint gradient (one forward + selected inversions)."""
         theta = np.asarray(theta, dtype=np.float64)
         parts = self.forward(theta)
-        eo = float(np.exp(theta[3]))
-        # quadratic terms need x*, taken before the factors are overwritten
+        # terms that need x*, taken before the factors are overwritten
         Zq = self._contract(None, None, self.x, 1)
-        xAtAx = self._bilinear(1, x=self.x)
+        xAtAx = [self._bilinear(o, 1, x=self.x) for o in self.obs]
+        xAty = [float(torch.dot(self.x, o.Aty)) for o in self.obs]
         self.cond.selected_inversion("bta")
         self.prior.selected_inversion("bt")
         Zc = self._contract(self.cond.A_diagonal_blocks, self.cond.A_lower_diagonal_blocks, None, 0)
         Zp = self._contract(self.prior.A_diagonal_blocks, self.prior.A_lower_diagonal_blocks, None, 0)
-        trAtA = self._bilinear(0, solver=self.cond)
-        J = self.coef.jacobian(theta)  # (d_st, nM, nK)
+        trAtA = [self._bilinear(o, 0, solver=self.cond) for o in self.obs]
+        J = self._jacobian(theta)
         g = np.zeros(theta.size)
-        dst = self.coef.d_st
-        for i in range(dst):
-            g[i] = 0.5 * np.sum(J[i] * Zc) - 0.5 * np.sum(J[i] * Zp) + 0.5 * np.sum(J[i] * Zq)
-        g[3] = 0.5 * eo * trAtA - parts["quad"] + 0.5 * eo * xAtAx - 0.5 * self.m_obs + 0.5 * eo * self.yty
+        Zsum = Zc - Zp + Zq
+        for i in range(J.shape[0]):
+            g[i] = 0.5 * np.sum(J[i] * Zsum)
+        for o, tr, q, xb in zip(self.obs, trAtA, xAtAx, xAty):
+            eo = float(np.exp(theta[o.idx]))
+            g[o.idx] += 0.5 * eo * tr - eo * xb + 0.5 * eo * q - 0.5 * o.m + 0.5 * eo * o.yty
         g -= gaussian_log_prior_grad(theta, self.prob.prior_mean, self.prob.prior_prec)
         return parts["f"], g
 

@@ -238,3 +238,75 @@ def st_problem(nt: int = 100, nx: int = 40, ny: int = 40, nb: int = 6, obs_per_n
     y = u[nodes] + X @ beta + noise
     return STProblem(ns=ns, nt=nt, nb=nb, spatial=spatial, temporal=temporal, A=A, y=y, theta=theta,
                      prior_mean=np.array([0.0, 0.0, 0.0, 1.0]), prior_prec=np.full(4, 0.5), fixed_prec=1e-3)
+
+
+@dataclass
+class CoregionalProblem:
+    """nv interleaved spatio-temporal fields on one mesh (coregional model).
+
+    Latent order: per time block t the fields 0..nv-1 (ns nodes each), i.e.
+    index t*nv*ns + v*ns + s (the reference's interleaving,
+    coregional_model.py:754-786), then the fixed effects (nb_per per field).
+    A_list[v], y_list[v]: observations of field v.
+    theta = (theta_s, theta_t, theta_e) per field, lambdas, noise precisions.
+    """
+    nv: int
+    ns: int
+    nt: int
+    nb: int
+    spatial: list
+    temporal: list
+    A_list: list
+    y_list: list
+    theta: np.ndarray
+    prior_mean: np.ndarray
+    prior_prec: np.ndarray
+    fixed_prec: float
+    A: object = None
+    y: object = None
+
+
+def coregional_problem(nv: int = 3, nt: int = 100, nx: int = 50, ny: int = 30, nb_per: int = 3,
+                       obs_per_node: int = 2, seed: int = 4) -> CoregionalProblem:
+    """Synthetic coregional model; default 3 fields of 1500 nodes (50 x 30 grid),
+    nt = 100, 3 fixed effects per field, 2 ns nt observations per field.
+
+    Draw order: (1) theta per field N(0, 0.1) x 3 (theta_s, theta_t, theta_e),
+    field by field; (2) lambdas N(0, 0.5); (3) noise precisions 1 + N(0, 0.1);
+    then for each field v: (4) observation nodes, (5) covariates N(0,1)
+    (m_v, nb_per), (6) latent field N(0, 0.5) (nt*ns,), (7) fixed effects
+    N(0,1), (8) noise N(0,1) * exp(-theta_o/2).
+    """
+    import scipy.sparse as sps
+    ns = nx * ny
+    spatial = grid_spatial_matrices(nx, ny, 1.0 / (nx - 1), 1.0 / (ny - 1))
+    temporal = temporal_matrices(nt)
+    rng = np.random.default_rng(seed)
+    n_lam = nv * (nv - 1) // 2
+    th_fields = rng.normal(0.0, 0.1, 3 * nv)
+    lam = rng.normal(0.0, 0.5, n_lam)
+    th_o = 1.0 + rng.normal(0.0, 0.1, nv)
+    theta = np.concatenate([th_fields, lam, th_o])
+    b = nv * ns
+    nb = nv * nb_per
+    N = nt * b + nb
+    A_list, y_list = [], []
+    for v in range(nv):
+        m = obs_per_node * ns * nt
+        nodes = rng.integers(0, ns * nt, size=m)
+        X = rng.normal(0.0, 1.0, (m, nb_per))
+        u = rng.normal(0.0, 0.5, ns * nt)
+        beta = rng.normal(0.0, 1.0, nb_per)
+        noise = rng.normal(0.0, 1.0, m) * np.exp(-0.5 * th_o[v])
+        t_idx, s_idx = nodes // ns, nodes % ns
+        col_st = t_idx * b + v * ns + s_idx
+        rows = np.repeat(np.arange(m), 1 + nb_per)
+        cols = np.concatenate([col_st[:, None], (nt * b + v * nb_per + np.arange(nb_per))[None, :].repeat(m, 0)],
+                              axis=1).ravel()
+        vals = np.concatenate([np.ones((m, 1)), X], axis=1).ravel()
+        A_list.append(sps.csr_matrix((vals, (rows, cols)), shape=(m, N)))
+        y_list.append(u[nodes] + X @ beta + noise)
+    prior_mean = np.concatenate([np.zeros(3 * nv + n_lam), np.ones(nv)])
+    return CoregionalProblem(nv=nv, ns=ns, nt=nt, nb=nb, spatial=spatial, temporal=temporal, A_list=A_list,
+                             y_list=y_list, theta=theta, prior_mean=prior_mean,
+                             prior_prec=np.full(theta.size, 0.5), fixed_prec=1e-3)

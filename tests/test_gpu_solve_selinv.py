@@ -84,3 +84,38 @@ def test_bt_solve_and_selinv_on_bta_buffers(cuda):
     ob.pobtsi(D, L)
     assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
     assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < RTOL
+
+
+def test_batched_factorization_mixed_shapes(cuda):
+    """Problems of different (n, b, a), BT and BTA, factored by one batched
+    graph: each must equal its single-problem oracle."""
+    from pyinla_amd.batch import BatchedFactorization
+    shapes = [(3, 92, 4, "bta"), (5, 64, 6, "bta"), (2, 130, 0, "bt"), (4, 70, 5, "bt"), (1, 17, 3, "bta")]
+    kfs, solvers, sps_ = [], [], []
+    for q, (n, b, a, sp) in enumerate(shapes):
+        kf = known_factor_bta(n, b, a, seed=300 + q)
+        s = SerinvSolver(None, b, n, a)
+        s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if a else None, kf.Q_T if a else None)
+        kfs.append(kf)
+        solvers.append(s)
+        sps_.append(sp)
+    batch = BatchedFactorization(solvers, sps_)
+    for rep in range(2):
+        if rep:
+            for kf, s in zip(kfs, solvers):
+                s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if kf.a else None, kf.Q_T if kf.a else None)
+        batch.factorize()
+        for (n, b, a, sp), kf, s in zip(shapes, kfs, solvers):
+            D, L = kf.Q_D.copy(), kf.Q_L.copy()
+            B = kf.Q_B.copy() if sp == "bta" and a else None
+            T = kf.Q_T.copy() if sp == "bta" and a else None
+            ob.pobtaf(D, L, B, T)
+            info, ld = s.status()
+            assert info == 0
+            ld_o = ob.logdet(D, T)
+            assert abs(ld - ld_o) <= 1e-10 * abs(ld_o)
+            assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
+            if n > 1:
+                assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < RTOL
+            if B is not None:
+                assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
