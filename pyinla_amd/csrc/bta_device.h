@@ -71,6 +71,43 @@ __device__ __forceinline__ void lds_load_tile_t(double* __restrict__ s, const do
   }
 }
 
+// A 64x64 operand tile staged global -> registers -> LDS, so that the global
+// loads of the next K chunk are in flight while the current chunk is
+// multiplied.  The LDS layout is always s[i][k] (row i of op(X), column k).
+//   trans == 0: op(X) = X, X row-major (rows x K): s[i][k] = X[i][k]
+//   trans == 1: op(X) = X^T, X row-major (K x rows): s[i][k] = X[k][i]
+struct TilePrefetch {
+  double r[16];
+  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw) {
+    const int c = threadIdx.x & 63;
+    const int r0 = threadIdx.x >> 6;
+    if (trans == 0) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int i = r0 + 4 * j;
+        r[j] = (i < rows && c < kw) ? g[(int64_t)i * ld + c] : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int k = r0 + 4 * j;
+        r[j] = (k < kw && c < rows) ? g[(int64_t)k * ld + c] : 0.0;
+      }
+    }
+  }
+  __device__ __forceinline__ void store(double* __restrict__ s, int trans) const {
+    const int c = threadIdx.x & 63;
+    const int r0 = threadIdx.x >> 6;
+    if (trans == 0) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) s[(r0 + 4 * j) * BTA_LDS_LD + c] = r[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) s[c * BTA_LDS_LD + r0 + 4 * j] = r[j];
+    }
+  }
+};
+
 // acc += sign * sA(64 x K) * sB(64 x K)^T over k in [0, kmax), kmax % 4 == 0.
 // sA[i][k] and sB[j][k] are LDS tiles with stride BTA_LDS_LD.
 template <bool NEG>

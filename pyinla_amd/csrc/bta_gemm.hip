@@ -72,23 +72,58 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __r
   } else {
     acc_zero(acc);
   }
-  for (int s = 0; s < g.nseg; ++s) {
-    const GemmSeg& sg = g.seg[s];
-    int kmin = 0, kmax = sg.k;
-    if (sg.tri & TRI_A_LOWER) kmax = min(kmax, (tr + 1) * 64);
-    if (sg.tri & TRI_A_UPPER) kmin = max(kmin, tr * 64);
-    if (sg.tri & TRI_B_LOWER) kmin = max(kmin, tc * 64);
-    if (sg.tri & TRI_B_UPPER) kmax = min(kmax, (tc + 1) * 64);
-    for (int k0 = kmin; k0 < kmax; k0 += 64) {
-      const int kw = min(64, kmax - k0);
-      if (sg.ta == 0) lds_load_tile(sA, sg.A + (int64_t)(tr * 64) * sg.lda + k0, sg.lda, rows, kw);
-      else lds_load_tile_t(sA, sg.A + (int64_t)k0 * sg.lda + tr * 64, sg.lda, kw, rows);
-      if (sg.tb == 1) lds_load_tile(sB, sg.B + (int64_t)(tc * 64) * sg.ldb + k0, sg.ldb, cols, kw);
-      else lds_load_tile_t(sB, sg.B + (int64_t)k0 * sg.ldb + tc * 64, sg.ldb, kw, cols);
-      __syncthreads();
-      mma_nt_alpha(acc, sA, sB, (kw + 3) & ~3, sg.alpha);
-      __syncthreads();
+  // K chunks of all segments form one pipeline: chunk q+1 is fetched into
+  // registers while chunk q is multiplied out of LDS.
+  int seg_id = 0, k0 = 0, kmax = 0;
+  auto seg_range = [&](int sid, int& kb, int& ke) {
+    const GemmSeg& sg = g.seg[sid];
+    kb = 0;
+    ke = sg.k;
+    if (sg.tri & TRI_A_LOWER) ke = min(ke, (tr + 1) * 64);
+    if (sg.tri & TRI_A_UPPER) kb = max(kb, tr * 64);
+    if (sg.tri & TRI_B_LOWER) kb = max(kb, tc * 64);
+    if (sg.tri & TRI_B_UPPER) ke = min(ke, (tc + 1) * 64);
+  };
+  // advance (seg_id, k0) to the next non-empty chunk; returns false at the end
+  auto next_chunk = [&](bool first) -> bool {
+    if (!first) k0 += 64;
+    while (seg_id < g.nseg) {
+      if (first || k0 >= kmax) {
+        if (!first) ++seg_id;
+        if (seg_id >= g.nseg) return false;
+        int kb;
+        seg_range(seg_id, kb, kmax);
+        k0 = kb;
+        first = false;
+        if (k0 < kmax) return true;
+        continue;
+      }
+      return true;
     }
+    return false;
+  };
+  TilePrefetch pa, pb;
+  auto fetch = [&]() {
+    const GemmSeg& sg = g.seg[seg_id];
+    const int kw = min(64, kmax - k0);
+    if (sg.ta == 0) pa.load(sg.A + (int64_t)(tr * 64) * sg.lda + k0, sg.lda, 0, rows, kw);
+    else pa.load(sg.A + (int64_t)k0 * sg.lda + tr * 64, sg.lda, 1, rows, kw);
+    if (sg.tb == 1) pb.load(sg.B + (int64_t)(tc * 64) * sg.ldb + k0, sg.ldb, 0, cols, kw);
+    else pb.load(sg.B + (int64_t)k0 * sg.ldb + tc * 64, sg.ldb, 1, cols, kw);
+  };
+  bool have = (g.nseg > 0) && next_chunk(true);
+  if (have) fetch();
+  while (have) {
+    const GemmSeg& sg = g.seg[seg_id];
+    const int kw = min(64, kmax - k0);
+    const double alpha = sg.alpha;
+    pa.store(sA, sg.ta);
+    pb.store(sB, sg.tb == 1 ? 0 : 1);
+    __syncthreads();
+    have = next_chunk(false);
+    if (have) fetch();
+    mma_nt_alpha(acc, sA, sB, (kw + 3) & ~3, alpha);
+    __syncthreads();
   }
   // epilogue
 #pragma unroll

@@ -293,11 +293,21 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_nt_sub(UpdJobs jobs) {
   const double* bt = g.B + (int64_t)(tc * 64) * g.ldb;
   TileAcc acc;
   acc_load(acc, ct, g.ldc, rows, cols);
+  TilePrefetch pa, pb;
+  if (g.kdim > 0) {
+    pa.load(at, g.lda, 0, rows, min(64, g.kdim));
+    pb.load(bt, g.ldb, 0, cols, min(64, g.kdim));
+  }
   for (int k0 = 0; k0 < g.kdim; k0 += 64) {
     const int kw = min(64, g.kdim - k0);
-    lds_load_tile(sA, at + k0, g.lda, rows, kw);
-    lds_load_tile(sB, bt + k0, g.ldb, cols, kw);
+    pa.store(sA, 0);
+    pb.store(sB, 0);
     __syncthreads();
+    if (k0 + 64 < g.kdim) {  // fetch the next chunk while this one is multiplied
+      const int kn = min(64, g.kdim - k0 - 64);
+      pa.load(at + k0 + 64, g.lda, 0, rows, kn);
+      pb.load(bt + k0 + 64, g.ldb, 0, cols, kn);
+    }
     mma_nt<true>(acc, sA, sB, 0, (kw + 3) & ~3);
     __syncthreads();
   }
@@ -328,7 +338,9 @@ __global__ void __launch_bounds__(BTA_THREADS) k_tip_potrf(double* T, int a, dou
 }
 
 // Ordered reduction of npart partial sums: thread t sums t, t+256, ... in
-// order, then a fixed pairwise tree.  out = 2 * sum.
+// order, then a fixed pairwise tree.  Each partial is a sum of log pivots,
+// and a pivot is the square of the factor's diagonal entry, so the plain sum
+// is the log-determinant 2 sum log diag L.
 __global__ void __launch_bounds__(BTA_THREADS) k_logdet_reduce(const double* part, int npart, double* out) {
   __shared__ double s[BTA_THREADS];
   double acc = 0.0;
@@ -339,7 +351,7 @@ __global__ void __launch_bounds__(BTA_THREADS) k_logdet_reduce(const double* par
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = 2.0 * s[0];
+  if (threadIdx.x == 0) out[0] = s[0];
 }
 
 __global__ void __launch_bounds__(BTA_THREADS) k_tip_potrf_batched(const ProbDesc* probs) {
@@ -375,7 +387,7 @@ __global__ void __launch_bounds__(BTA_THREADS) k_logdet_reduce_batched(const Pro
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) d.logdet[0] = 2.0 * s[0];
+  if (threadIdx.x == 0) d.logdet[0] = s[0];
 }
 
 // ------------------------------------------------------------ layout check

@@ -133,17 +133,21 @@ class STGaussianModel:
         for s in S[1:]:
             union = union + abs(s)
         union = sps.csr_matrix(union)
+        union.sum_duplicates()
         union.sort_indices()
+        urow = np.repeat(np.arange(ns, dtype=np.int64), np.diff(union.indptr))
+        ukey = urow * ns + union.indices.astype(np.int64)  # sorted (CSR, sorted indices)
         vals = []
         for s in S:
-            s2 = s.tocsr().copy()
-            s2.sort_indices()
-            pat = union.copy()
-            pat.data[:] = 0.0
-            v = (pat + s2).tocsr()
-            v.sort_indices()
-            assert v.nnz == union.nnz  # union contains every entry of s2
-            vals.append(v.data.copy())
+            c = sps.coo_matrix(s)
+            c.sum_duplicates()
+            key = c.row.astype(np.int64) * ns + c.col.astype(np.int64)
+            pos = np.searchsorted(ukey, key)
+            if not np.array_equal(ukey[np.minimum(pos, len(ukey) - 1)], key):
+                raise ValueError("spatial matrix entry outside the union pattern")
+            v = np.zeros(len(ukey))
+            v[pos] = c.data
+            vals.append(v)
         self.nnz = union.nnz
         self.rowptr = torch.from_numpy(union.indptr.astype(np.int32)).to(dev)
         self.colidx = torch.from_numpy(union.indices.astype(np.int32)).to(dev)
