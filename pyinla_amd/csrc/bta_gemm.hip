@@ -150,8 +150,8 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __r
 // ------------------------------------------------------------------- GEMV
 //
 // N form (op(A) = A): one wave per output row, lanes stride the columns,
-// butterfly reduction.  T form (op(A) = A^T): 64 output columns per
-// workgroup, the 4 waves sum interleaved rows, fixed-order combine.
+// butterfly reduction.  T form (op(A) = A^T): 16 output columns per
+// workgroup, 16 row groups sum interleaved rows, fixed-order combine.
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -183,9 +183,12 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemv(GemvJob g, int form_t) {
       }
     }
   } else {
-    __shared__ double red[4][64];
-    const int j = blockIdx.x * 64 + lane;
-    for (int c = 0; c < g.nrhs; ++c) {
+    // 16 output columns per workgroup; thread (c = t%16, g = t/16) sums rows
+    // k = g, g+16, ... of column j, then a fixed-order combine over g.
+    __shared__ double red[16][17];
+    const int c = threadIdx.x & 15, grp = threadIdx.x >> 4;
+    const int j = blockIdx.x * 16 + c;
+    for (int rc = 0; rc < g.nrhs; ++rc) {
       double acc = 0.0;
       for (int s = 0; s < g.nseg; ++s) {
         const GemvSeg& sg = g.seg[s];
@@ -194,16 +197,20 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemv(GemvJob g, int form_t) {
           int kmin = 0, kmax = sg.k;
           if (sg.tri == 1) kmax = min(kmax, j + 1);
           if (sg.tri == 2) kmin = j;
-          for (int kk = kmin + w; kk < kmax; kk += 4)
-            part = fma(sg.A[(int64_t)kk * sg.lda + j], sg.x[(int64_t)kk * sg.ldx + c], part);
+          for (int kk = kmin + grp; kk < kmax; kk += 16)
+            part = fma(sg.A[(int64_t)kk * sg.lda + j], sg.x[(int64_t)kk * sg.ldx + rc], part);
         }
-        red[w][lane] = part;
+        red[grp][c] = part;
         __syncthreads();
-        if (w == 0) acc = fma(sg.alpha, ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane])), acc);
+        if (grp == 0) {
+          double t = 0.0;
+          for (int q = 0; q < 16; ++q) t += red[q][c];
+          acc = fma(sg.alpha, t, acc);
+        }
         __syncthreads();
       }
-      if (w == 0 && j < g.m) {
-        double* yp = g.y + (int64_t)j * g.ldy + c;
+      if (grp == 0 && j < g.m) {
+        double* yp = g.y + (int64_t)j * g.ldy + rc;
         *yp = (g.beta != 0.0) ? fma(g.beta, *yp, acc) : acc;
       }
     }
@@ -256,7 +263,7 @@ void DevPlan::launch(int idx, hipStream_t st) const {
 
 void gemv_launch(const GemvJob& g, int form_t, hipStream_t st) {
   if (g.m <= 0) return;
-  const int grid = form_t ? (g.m + 63) / 64 : (g.m + 3) / 4;
+  const int grid = form_t ? (g.m + 15) / 16 : (g.m + 3) / 4;
   hipLaunchKernelGGL(k_gemv, dim3(grid), dim3(BTA_THREADS), 0, st, g, form_t);
 }
 

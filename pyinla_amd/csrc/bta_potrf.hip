@@ -180,7 +180,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* 
   acc_store(x, ct, b, rows, cols);
 }
 
-__global__ void __launch_bounds__(BTA_THREADS) k_panel_step(PanelArgs p) {
+__global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_step(PanelArgs p) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
   __shared__ double sScr[4 * 64 + 64];
@@ -215,7 +215,7 @@ __device__ __forceinline__ PanelArgs panel_args_of(const ProbDesc& d, int i, int
   return p;
 }
 
-__global__ void __launch_bounds__(BTA_THREADS) k_panel_step_batched(const ProbDesc* __restrict__ probs,
+__global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_step_batched(const ProbDesc* __restrict__ probs,
                                                                    const int* __restrict__ ent_prob,
                                                                    const int* __restrict__ ent_begin, int nent,
                                                                    int i, int k) {

@@ -184,7 +184,13 @@ class STGaussianModel:
         chunks = (ns + 63) // 64
         self.work = torch.zeros(max(nt * nv * chunks * 2 * nv * self.nK, 4096), dtype=torch.float64, device=dev)
         self.contract_out = torch.zeros(nt * nv * nv * self.nK * 2, dtype=torch.float64, device=dev)
+        self.work_p = torch.zeros_like(self.work)
+        self.contract_out_p = torch.zeros_like(self.contract_out)
         self.scalar = torch.zeros(4, dtype=torch.float64, device=dev)
+        # the conditional and the prior chains are independent: each gets a
+        # stream of its own (default priority) and they overlap on the GPU
+        self.s_cond = torch.cuda.Stream(device=dev)
+        self.s_prior = torch.cuda.Stream(device=dev)
 
     # ---------------------------------------------------------- assembly
 
@@ -235,18 +241,34 @@ class STGaussianModel:
 
     # -------------------------------------------------------------- f(theta)
 
+    def _fork(self):
+        cur = torch.cuda.current_stream(self.device)
+        self.s_cond.wait_stream(cur)
+        self.s_prior.wait_stream(cur)
+        return cur
+
+    def _join(self, cur):
+        cur.wait_stream(self.s_cond)
+        cur.wait_stream(self.s_prior)
+
     def forward(self, theta) -> dict:
-        """Assemble, factor both matrices, solve for x*; returns the pieces of f."""
+        """Assemble, factor both matrices (on two streams), solve for x*;
+        returns the pieces of f."""
         theta = np.asarray(theta, dtype=np.float64)
-        self.assemble_cond(theta)
-        self.cond.factorize("bta")
-        self.assemble_prior(theta)
-        self.prior.factorize("bt")
-        self._rhs(theta)
-        self.x.copy_(self.rhs)
-        self.cond.solve(self.x, "bta")
-        info_c, ld_c = self.cond.status()
-        info_p, ld_p = self.prior.status()
+        cur = self._fork()
+        with torch.cuda.stream(self.s_prior):
+            self.assemble_prior(theta)
+            self.prior.factorize("bt")
+        with torch.cuda.stream(self.s_cond):
+            self.assemble_cond(theta)
+            self.cond.factorize("bta")
+            self._rhs(theta)
+            self.x.copy_(self.rhs)
+            self.cond.solve(self.x, "bta")
+            info_c, ld_c = self.cond.status()
+        with torch.cuda.stream(self.s_prior):
+            info_p, ld_p = self.prior.status()
+        self._join(cur)
         if info_c or info_p:
             raise np.linalg.LinAlgError(f"not positive definite (cond info {info_c}, prior info {info_p})")
         quad = float(torch.dot(self.x, self.rhs))
@@ -314,19 +336,30 @@ class STGaussianModel:
 
     # ---------------------------------------------------------- adjoint
 
-    def _contract(self, D, L, x, mode):
-        """Z[v][w][m][k] = sum_t Mdiag[m][t] PD[t][v][w][k] + 2 sum_t Moff[m][t] PL[t][v][w][k]."""
+    def _contract_launch(self, D, L, x, mode, work=None, out=None):
+        """Enqueue the Kronecker contraction on the current stream; returns the
+        device output (n, nv, nv, nK, 2)."""
         n, b, nv, nK = self.n, self.b, self.nv, self.nK
+        work = self.work if work is None else work
+        out_t = self.contract_out if out is None else out
         _lib.check(self.lib.bta_kron_contract(
             _ptr(D), _ptr(L), _ptr(x), n, b, nv, _ptr(self.rowptr), _ptr(self.colidx), _ptr(self.vals),
-            self.nnz, nK, mode, _ptr(self.work), _ptr(self.contract_out), _stream_ptr(self.device)),
+            self.nnz, nK, mode, _ptr(work), _ptr(out_t), _stream_ptr(self.device)),
             "bta_kron_contract")
-        out = self.contract_out[: n * nv * nv * nK * 2].reshape(n, nv, nv, nK, 2).cpu().numpy()
+        return out_t[: n * nv * nv * nK * 2].reshape(n, nv, nv, nK, 2)
+
+    def _contract_finish(self, out_dev):
+        """Z[v][w][m][k] = sum_t Mdiag[m][t] PD[t][v][w][k] + 2 sum_t Moff[m][t] PL[t][v][w][k]."""
+        n = self.n
+        out = out_dev.cpu().numpy()
         PD, PL = out[..., 0], out[..., 1]
         Z = np.einsum("mt,tvwk->vwmk", self.Mdiag_np, PD)
         if n > 1:
             Z = Z + 2.0 * np.einsum("mt,tvwk->vwmk", self.Moff_np, PL[: n - 1])
         return Z
+
+    def _contract(self, D, L, x, mode, work=None, out=None):
+        return self._contract_finish(self._contract_launch(D, L, x, mode, work, out))
 
     def _bilinear(self, o: _ObsSet, mode, solver=None, x=None):
         s = solver
@@ -347,10 +380,17 @@ class STGaussianModel:
         Zq = self._contract(None, None, self.x, 1)
         xAtAx = [self._bilinear(o, 1, x=self.x) for o in self.obs]
         xAty = [float(torch.dot(self.x, o.Aty)) for o in self.obs]
-        self.cond.selected_inversion("bta")
-        self.prior.selected_inversion("bt")
-        Zc = self._contract(self.cond.A_diagonal_blocks, self.cond.A_lower_diagonal_blocks, None, 0)
-        Zp = self._contract(self.prior.A_diagonal_blocks, self.prior.A_lower_diagonal_blocks, None, 0)
+        cur = self._fork()
+        with torch.cuda.stream(self.s_prior):
+            self.prior.selected_inversion("bt")
+            op = self._contract_launch(self.prior.A_diagonal_blocks, self.prior.A_lower_diagonal_blocks, None, 0,
+                                       work=self.work_p, out=self.contract_out_p)
+        with torch.cuda.stream(self.s_cond):
+            self.cond.selected_inversion("bta")
+            oc = self._contract_launch(self.cond.A_diagonal_blocks, self.cond.A_lower_diagonal_blocks, None, 0)
+        self._join(cur)
+        Zp = self._contract_finish(op)
+        Zc = self._contract_finish(oc)
         trAtA = [self._bilinear(o, 0, solver=self.cond) for o in self.obs]
         J = self._jacobian(theta)
         g = np.zeros(theta.size)
