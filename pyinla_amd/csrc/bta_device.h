@@ -184,33 +184,46 @@ __device__ __forceinline__ void acc_to_lds(const TileAcc& t, double* __restrict_
 // ------------------------------------------------------------ cross-WG flags
 //
 // Flags are 64-bit words holding the call epoch when the guarded data is ready.
-// The producer stores its data, every thread fences at agent scope (the L2s of
-// the XCDs are not coherent with each other), then one thread publishes the
-// flag with a release store.  A consumer spins (bounded) on an acquire load.
+// The XCDs' L2 caches are not coherent with each other, and agent-scope
+// release / acquire fences write back / invalidate a whole L2.  The data a
+// flag guards is therefore written with system-scope relaxed atomic stores
+// (write-through to the coherence point) and read with system-scope relaxed
+// atomic loads (which bypass the non-coherent caches); the producer waits for
+// its stores to complete before it stores the flag the same way, and the
+// consumer polls the flag with the same cache-bypassing loads.
 
+__device__ __forceinline__ void st_sys(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ double ld_sys(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// All threads call after their st_sys stores of the guarded data.
 __device__ __forceinline__ void flag_publish(unsigned long long* flag, unsigned long long epoch) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's stores have completed
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __syncthreads();                // ... and every other thread's
+  if (threadIdx.x == 0) __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Returns false if the spin limit was hit (the caller records an error and
-// carries on so that every wave still exits).
+// carries on so that every wave still exits).  Guarded data must then be read
+// with ld_sys.
 __device__ __forceinline__ bool flag_wait(unsigned long long* flag, unsigned long long epoch,
                                           unsigned long long spin_limit) {
   __shared__ int ok_sh;
   if (threadIdx.x == 0) {
     int ok = 1;
     unsigned long long it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != epoch) {
       if (++it > spin_limit) { ok = 0; break; }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
     ok_sh = ok;
   }
   __syncthreads();
-  // Every thread performs its own acquire so its later loads are ordered.
-  (void)__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   const int ok = ok_sh;
   __syncthreads();
   return ok != 0;

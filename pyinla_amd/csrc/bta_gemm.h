@@ -45,15 +45,21 @@ struct GemmJob {
 struct GemmLaunch {
   std::vector<GemmJob> jobs;
   std::vector<int> prefix;  // njobs + 1
+  int ts = 64;              // output tile edge: 64 or 128
+  explicit GemmLaunch(int tile = 64) : ts(tile) {}
   int tiles() const { return prefix.empty() ? 0 : prefix.back(); }
   void add(const GemmJob& j) {
     if (j.m <= 0 || j.n <= 0) return;
     if (prefix.empty()) prefix.push_back(0);
-    const int tm = (j.m + 63) / 64, tn = (j.n + 63) / 64;
+    const int tm = (j.m + ts - 1) / ts, tn = (j.n + ts - 1) / ts;
     jobs.push_back(j);
     prefix.push_back(prefix.back() + (j.lower ? tm * (tm + 1) / 2 : tm * tn));
   }
 };
+
+// Output tile edge for a GEMM whose output is m x n: 128 when the output is
+// large enough to keep the chip busy with 128x128 tiles, else 64.
+inline int pick_tile(int m, int n) { return (m >= 512 && n >= 512) ? 128 : 64; }
 
 inline GemmSeg seg(const double* A, int lda, int ta, const double* B, int ldb, int tb, int k,
                    double alpha, int tri = 0) {
@@ -113,7 +119,7 @@ struct GemvJob {
 // once (hipMalloc + hipMemcpy, never during stream capture), then launched any
 // number of times, including from inside a captured hipGraph.
 struct DevPlan {
-  struct Entry { int job_off, pre_off, njobs, tiles; };
+  struct Entry { int job_off, pre_off, njobs, tiles, ts; };
   GemmJob* d_jobs = nullptr;
   int* d_prefix = nullptr;
   std::vector<Entry> launches;

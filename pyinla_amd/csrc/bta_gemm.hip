@@ -147,6 +147,160 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __r
       }
 }
 
+// ------------------------------------------------- 128x128 output tiles
+//
+// Same job semantics as k_gemm_grouped with 128x128 output tiles: wave w owns
+// the 64x64 quadrant (64*(w/2), 64*(w%2)) as 4x4 MFMA 16x16 accumulators, K
+// advances in chunks of 16 staged through LDS, the next chunk is fetched into
+// registers while the current one is multiplied.
+
+#define BIG_TS 128
+#define BIG_KC 16
+#define BIG_LD (BIG_KC + 2)  // 18 doubles: 16 consecutive rows hit distinct LDS banks
+
+struct BigOp {
+  double r[BIG_TS * BIG_KC / BTA_THREADS];  // 8
+  // trans == 0: X row-major rows x K -> element (i, k) = X[i][k]
+  // trans == 1: X row-major K x rows -> element (i, k) = X[k][i]
+  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int e = threadIdx.x + BTA_THREADS * p;
+      if (trans == 0) {
+        const int i = e >> 4, k = e & 15;
+        r[p] = (i < rows && k < kw) ? g[(int64_t)i * ld + k] : 0.0;
+      } else {
+        const int k = e >> 7, i = e & 127;
+        r[p] = (i < rows && k < kw) ? g[(int64_t)k * ld + i] : 0.0;
+      }
+    }
+  }
+  __device__ __forceinline__ void store(double* __restrict__ s, int trans) const {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+      const int e = threadIdx.x + BTA_THREADS * p;
+      if (trans == 0) s[(e >> 4) * BIG_LD + (e & 15)] = r[p];
+      else s[(e & 127) * BIG_LD + (e >> 7)] = r[p];
+    }
+  }
+};
+
+__global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped_big(const GemmJob* __restrict__ jobs,
+                                                                 const int* __restrict__ prefix, int njobs) {
+  __shared__ double sA[BIG_TS * BIG_LD];
+  __shared__ double sB[BIG_TS * BIG_LD];
+  const int jid = find_job(prefix, njobs, blockIdx.x);
+  const GemmJob& g = jobs[jid];
+  const int local = blockIdx.x - prefix[jid];
+  int tr, tc;
+  if (g.lower) decode_lower(local, tr, tc);
+  else {
+    const int tn = (g.n + BIG_TS - 1) / BIG_TS;
+    tr = local / tn;
+    tc = local % tn;
+  }
+  const int rows = min(BIG_TS, g.m - tr * BIG_TS), cols = min(BIG_TS, g.n - tc * BIG_TS);
+  double* ct = g.C + (int64_t)(tr * BIG_TS) * g.ldc + tc * BIG_TS;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      if (g.beta != 0.0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = wm + 16 * x + 4 * r + lk, j = wn + 16 * y + li;
+          acc[x][y][r] = (i < rows && j < cols) ? g.beta * ct[(int64_t)i * g.ldc + j] : 0.0;
+        }
+      } else {
+        acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
+      }
+    }
+  int seg_id = 0, k0 = 0, kmax = 0;
+  auto seg_range = [&](int sid, int& kb, int& ke) {
+    const GemmSeg& sg = g.seg[sid];
+    kb = 0;
+    ke = sg.k;
+    if (sg.tri & TRI_A_LOWER) ke = min(ke, (tr + 1) * BIG_TS);
+    if (sg.tri & TRI_A_UPPER) kb = max(kb, tr * BIG_TS);
+    if (sg.tri & TRI_B_LOWER) kb = max(kb, tc * BIG_TS);
+    if (sg.tri & TRI_B_UPPER) ke = min(ke, (tc + 1) * BIG_TS);
+  };
+  auto next_chunk = [&](bool first) -> bool {
+    if (!first) k0 += BIG_KC;
+    while (seg_id < g.nseg) {
+      if (first || k0 >= kmax) {
+        if (!first) ++seg_id;
+        if (seg_id >= g.nseg) return false;
+        int kb;
+        seg_range(seg_id, kb, kmax);
+        k0 = kb;
+        first = false;
+        if (k0 < kmax) return true;
+        continue;
+      }
+      return true;
+    }
+    return false;
+  };
+  BigOp pa, pb;
+  auto fetch = [&]() {
+    const GemmSeg& sg = g.seg[seg_id];
+    const int kw = min(BIG_KC, kmax - k0);
+    if (sg.ta == 0) pa.load(sg.A + (int64_t)(tr * BIG_TS) * sg.lda + k0, sg.lda, 0, rows, kw);
+    else pa.load(sg.A + (int64_t)k0 * sg.lda + tr * BIG_TS, sg.lda, 1, rows, kw);
+    if (sg.tb == 1) pb.load(sg.B + (int64_t)(tc * BIG_TS) * sg.ldb + k0, sg.ldb, 0, cols, kw);
+    else pb.load(sg.B + (int64_t)k0 * sg.ldb + tc * BIG_TS, sg.ldb, 1, cols, kw);
+  };
+  bool have = (g.nseg > 0) && next_chunk(true);
+  if (have) fetch();
+  while (have) {
+    const GemmSeg& sg = g.seg[seg_id];
+    const int kw = min(BIG_KC, kmax - k0);
+    const double alpha = sg.alpha;
+    const int sta = sg.ta, stb = (sg.tb == 1) ? 0 : 1;
+    pa.store(sA, sta);
+    pb.store(sB, stb);
+    __syncthreads();
+    have = next_chunk(false);
+    if (have) fetch();
+    for (int k = 0; k < kw; k += 4) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) av[x] = alpha * sA[(wm + 16 * x + li) * BIG_LD + k + lk];
+#pragma unroll
+      for (int y = 0; y < 4; ++y) bv[y] = sB[(wn + 16 * y + li) * BIG_LD + k + lk];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = mfma_f64(av[x], bv[y], acc[x][y]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = wm + 16 * x + 4 * r + lk, j = wn + 16 * y + li;
+        if (i >= rows || j >= cols) continue;
+        const double v = acc[x][y][r];
+        if (g.lower && tr == tc) {
+          if (i >= j) {
+            ct[(int64_t)i * g.ldc + j] = v;
+            if (g.mirror && i > j) ct[(int64_t)j * g.ldc + i] = v;
+          }
+        } else {
+          ct[(int64_t)i * g.ldc + j] = v;
+          if (g.lower && g.mirror) g.C[(int64_t)(tc * BIG_TS + j) * g.ldc + tr * BIG_TS + i] = v;
+        }
+      }
+}
+
 // ------------------------------------------------------------------- GEMV
 //
 // N form (op(A) = A): one wave per output row, lanes stride the columns,
@@ -239,6 +393,7 @@ int DevPlan::build(const std::vector<GemmLaunch>& ls) {
     e.pre_off = (int)pre.size();
     e.njobs = (int)l.jobs.size();
     e.tiles = l.tiles();
+    e.ts = l.ts;
     launches.push_back(e);
     all.insert(all.end(), l.jobs.begin(), l.jobs.end());
     pre.insert(pre.end(), l.prefix.begin(), l.prefix.end());
@@ -257,8 +412,12 @@ int DevPlan::build(const std::vector<GemmLaunch>& ls) {
 void DevPlan::launch(int idx, hipStream_t st) const {
   const Entry& e = launches[idx];
   if (e.tiles <= 0) return;
-  hipLaunchKernelGGL(k_gemm_grouped, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
-                     d_prefix + e.pre_off, e.njobs);
+  if (e.ts == BIG_TS)
+    hipLaunchKernelGGL(k_gemm_grouped_big, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
+                       d_prefix + e.pre_off, e.njobs);
+  else
+    hipLaunchKernelGGL(k_gemm_grouped, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
+                       d_prefix + e.pre_off, e.njobs);
 }
 
 void gemv_launch(const GemvJob& g, int form_t, hipStream_t st) {

@@ -149,7 +149,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* 
       double* wk = p.Winv + (int64_t)k * 4096;
       for (int r = threadIdx.x >> 6; r < 64; r += 4) {
         if (r < mv && c < mv) ct[(int64_t)r * b + c] = sA[r * BTA_LDS_LD + c];
-        wk[r * 64 + c] = sB[r * BTA_LDS_LD + c];
+        st_sys(&wk[r * 64 + c], sB[r * BTA_LDS_LD + c]);  // read by this launch's TRSM tiles
       }
     }
     if (threadIdx.x == 0) {
@@ -169,7 +169,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* 
   {
     const double* wk = p.Winv + (int64_t)k * 4096;
     const int c = threadIdx.x & 63;
-    for (int r = threadIdx.x >> 6; r < 64; r += 4) sB[r * BTA_LDS_LD + c] = wk[r * 64 + c];
+    for (int r = threadIdx.x >> 6; r < 64; r += 4) sB[r * BTA_LDS_LD + c] = ld_sys(&wk[r * 64 + c]);
   }
   __syncthreads();
   TileAcc x;
@@ -484,6 +484,9 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps) {
   std::vector<int> ent;  // per launch: [prob..., begin..., total]
   launches.clear();
   std::vector<GemmLaunch> upd;
+  int bmin = 1 << 30;
+  for (const auto& d : ps) bmin = std::min(bmin, d.b);
+  const int ts = pick_tile(bmin, bmin);
   for (int i = 0; i < nmax; ++i) {
     int ntmax = 0;
     for (const auto& d : ps)
@@ -513,7 +516,7 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps) {
       L.update = -1;
       launches.push_back(L);
     }
-    GemmLaunch g;
+    GemmLaunch g(ts);
     for (const auto& d : ps) {
       if (i >= d.n) continue;
       const int b = d.b, a = d.a, n = d.n;

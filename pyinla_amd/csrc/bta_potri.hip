@@ -50,7 +50,8 @@ int trtri_prepare(Workspace& ws, const double* D, int n, int b) {
   const int nT = (b + 63) / 64;
   std::vector<GemmLaunch> ls;
   for (int s = 1; s < nT; s *= 2) {
-    GemmLaunch la, lb;
+    const int ts = pick_tile(s * 64, s * 64);
+    GemmLaunch la(ts), lb(ts);
     for (int t0 = 0; t0 + s < nT; t0 += 2 * s) {
       const int r1 = t0 * 64, r2 = (t0 + s) * 64;
       const int m1 = s * 64;
@@ -102,6 +103,7 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
   double* G = ws.work;                      // b x b
   double* Ga = ws.work + (size_t)b * b;     // a x b
   std::vector<GemmLaunch> ls;
+  const int ts = pick_tile(b, b);
   if (a > 0) {  // Sigma_T = Tinv^T Tinv
     GemmLaunch l0;
     GemmJob j = job(T, a, a, a, 0.0);
@@ -117,7 +119,7 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     double* Bi = a > 0 ? B + (int64_t)i * a * b : nullptr;
     double* Dn = hasL ? D + (int64_t)(i + 1) * b * b : nullptr;
     double* Bn = (hasL && a > 0) ? B + (int64_t)(i + 1) * a * b : nullptr;
-    GemmLaunch l1, l2, l3;
+    GemmLaunch l1(ts), l2(ts), l3(ts);
     if (hasL) {
       GemmJob j = job(G, b, b, b, 0.0);
       add_seg(j, seg(Li, b, 0, Wi, b, 0, b, 1.0, TRI_B_LOWER));
