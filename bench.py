@@ -69,6 +69,13 @@ def parse():
                     help="first run a plain device-to-device copy of MIB MiB (a known-size kernel to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters against)")
     ap.add_argument("--skip-fd", action="store_true", help="skip the FD comparison (profiling runs)")
+    ap.add_argument("--workload", default="adjoint",
+                    choices=["adjoint", "fd-batch", "mixed-batch", "coregional-lbfgs"],
+                    help="adjoint: the headline line; the others measure the batched FD stencil, the "
+                         "mixed-size batch and coregional L-BFGS (their own JSON lines)")
+    ap.add_argument("--fd-wave", type=int, default=0, help="stencil points per batched FD wave (0 = all)")
+    ap.add_argument("--lbfgs-iters", type=int, default=20)
+    ap.add_argument("--lbfgs-fd-iters", type=int, default=1)
     return ap.parse_args()
 
 
@@ -114,6 +121,116 @@ def cpu_baseline(args, prob_full):
             "sample_f_seconds": t_f, "extrapolated_fd_grad_seconds": t_grad_full}
 
 
+def bench_fd_batch(args, dev):
+    """2 dim(theta) + 1 = 9 FD evaluations of the nt x ns model: sequential
+    (the reference's loop) against all 18 factorizations in one batched graph."""
+    import torch
+    from pyinla_amd.st_model import STGaussianModel
+    from pyinla_amd.workloads import st_problem
+    prob = st_problem(nt=args.nt, nx=args.nx, ny=args.ny, nb=args.nb, seed=2)
+    n, b, a = prob.nt, prob.ns, prob.nb
+    model = STGaussianModel(prob, device=dev)
+    th = prob.theta.copy()
+    wave = args.fd_wave or None
+    model.fd_gradient(th)
+    model.fd_gradient_batched(th, wave=wave)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f1, g1 = model.fd_gradient(th)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    f2, g2 = model.fd_gradient_batched(th, wave=wave)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    fl = 9 * (flops_potrf(n, b, a) + flops_potrf(n, b, 0))
+    return {"metric": "FD f+grad (9 evaluations, 18 factorizations) wall time", "unit": "s",
+            "config": {"workload": f"gaussian spatio-temporal FD stencil nt={n} ns={b} nb={a}", "wave": wave or 9},
+            "sequential_s": t_seq, "batched_s": t_bat, "speedup": t_seq / t_bat,
+            "batched_factorization_tflops_incl_solves": fl / t_bat / 1e12,
+            "grad_max_rel_diff": float(np.abs(g1 - g2).max() / np.abs(g1).max()),
+            "f_rel_diff": abs(f1 - f2) / abs(f1)}
+
+
+def bench_mixed_batch(args, dev):
+    """24 known-factor BTA matrices, nt in {50,100,200}, b in {256,...,2048}
+    (seeded draw), a = 6: one batched graph against sequential factorizations."""
+    import torch
+    from pyinla_amd.batch import BatchedFactorization
+    from pyinla_amd.solver import SerinvSolver
+    from pyinla_amd.workloads import known_factor_bta_torch
+    rng = np.random.default_rng(2)
+    shapes = [(int(rng.choice([50, 100, 200])), int(rng.choice([256, 512, 1024, 2048]))) for _ in range(24)]
+    a = 6
+    solvers, Qs, total = [], [], 0.0
+    for q, (n, b) in enumerate(shapes):
+        _, Q = known_factor_bta_torch(n, b, a, seed=100 + q, device=dev)
+        s = SerinvSolver(None, b, n, a, device=dev)
+        Qs.append(Q)
+        solvers.append(s)
+        total += flops_potrf(n, b, a)
+
+    def load_all():
+        for s, Q in zip(solvers, Qs):
+            s.load_blocks(*Q)
+    batch = BatchedFactorization(solvers, ["bta"] * len(solvers))
+    load_all()
+    batch.factorize()
+    for s in solvers:
+        s.factorize("bta")
+    torch.cuda.synchronize()
+    load_all()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.factorize()
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    ld_b = [s.status()[1] for s in solvers]
+    load_all()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.factorize("bta")
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    ld_s = [s.status()[1] for s in solvers]
+    return {"metric": "mixed batch of 24 BTA factorizations", "unit": "TFLOP/s",
+            "config": {"workload": "24 known-factor BTA matrices, nt in {50,100,200}, b in {256,512,1024,2048}, a=6",
+                       "shapes": shapes},
+            "batched_s": t_bat, "sequential_s": t_seq, "batched_tflops": total / t_bat / 1e12,
+            "sequential_tflops": total / t_seq / 1e12,
+            "max_logdet_rel_diff": max(abs(x - y) / abs(y) for x, y in zip(ld_b, ld_s))}
+
+
+def bench_coregional(args, dev):
+    """Trivariate coregional model (3 x 1500 nodes interleaved, b = 4500,
+    nt = 100, 9 fixed effects, dim theta = 15): L-BFGS-B with the adjoint
+    gradient; the FD (31 evaluations) run is timed for a few iterations."""
+    import torch
+    from pyinla_amd.optimize import minimize
+    from pyinla_amd.st_model import STGaussianModel
+    from pyinla_amd.workloads import coregional_problem
+    prob = coregional_problem(nv=3, nt=args.nt, nx=50, ny=30, nb_per=3, obs_per_node=2, seed=4)
+    model = STGaussianModel(prob, device=dev)
+    th0 = prob.theta.copy()
+    model.value_and_grad(th0)
+    torch.cuda.synchronize()
+    r_adj = minimize(model, th0, gradient="adjoint", max_iter=args.lbfgs_iters)
+    torch.cuda.synchronize()
+    r_fd = minimize(model, th0, gradient="fd", max_iter=args.lbfgs_fd_iters)
+    torch.cuda.synchronize()
+    per_it_adj = r_adj["wall_time"] / max(r_adj["iterations"], 1)
+    per_it_fd = r_fd["wall_time"] / max(r_fd["iterations"], 1)
+    return {"metric": "coregional L-BFGS wall time per iteration", "unit": "s",
+            "config": {"workload": f"coregional nv=3 ns=1500 nt={args.nt} b=4500 nb=9 dim_theta=15"},
+            "adjoint": {"iterations": r_adj["iterations"], "objective_calls": r_adj["objective_calls"],
+                        "wall_s": r_adj["wall_time"], "s_per_iter": per_it_adj, "f": r_adj["f"],
+                        "message": r_adj["message"]},
+            "fd": {"iterations": r_fd["iterations"], "objective_calls": r_fd["objective_calls"],
+                   "wall_s": r_fd["wall_time"], "s_per_iter": per_it_fd, "f": r_fd["f"]},
+            "fd_over_adjoint_per_iter": per_it_fd / per_it_adj}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -128,6 +245,11 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
     lib = _lib.load()
+    if args.workload != "adjoint":
+        fn = {"fd-batch": bench_fd_batch, "mixed-batch": bench_mixed_batch,
+              "coregional-lbfgs": bench_coregional}[args.workload]
+        print(json.dumps(fn(args, dev)), flush=True)
+        return
     if args.copy_calibration:
         nel = args.copy_calibration * (1 << 20) // 8
         src = torch.ones(nel, dtype=torch.float64, device=dev)

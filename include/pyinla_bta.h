@@ -94,6 +94,15 @@ int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* st
  * (valid after the stream reaches that point; no host synchronisation). */
 double* bta_ws_logdet_ptr(bta_ws_t ws);
 
+/* Tuning options (no reference counterpart): the output-tile edge (64 or
+ * 128; 0 = built-in heuristic) of the FP64-MFMA GEMMs used for the block
+ * updates of bta_potrf (BTA_OPT_TILE_UPDATE) and for the block inverses and
+ * selected inversion (BTA_OPT_TILE_GROUPED).  Set by the autotuner
+ * (pyinla_amd/autotune.py); recorded graphs are dropped. */
+#define BTA_OPT_TILE_UPDATE 1
+#define BTA_OPT_TILE_GROUPED 2
+int bta_ws_set_option(bta_ws_t ws, int option, int value);
+
 /* Optional event timing (no reference counterpart; the reference times whole
  * calls with perf_counter, structured_solver.py:80-100).  While on, graphs are
  * bypassed and every launch of the classes 0 = panel factorization step,

@@ -27,6 +27,7 @@ SIGNATURES = {
     "bta_batch_create": (_I, [_I, _P, _P, _P, _P, _P, _P, _I, C.POINTER(_P)]),
     "bta_batch_potrf": (_I, [_P, _P]),
     "bta_batch_destroy": (_I, [_P]),
+    "bta_ws_set_option": (_I, [_P, _I, _I]),
     "bta_ws_set_timing": (_I, [_P, _I]),
     "bta_ws_reset_timing": (_I, [_P]),
     "bta_ws_get_timing": (_I, [_P, _I, C.POINTER(_D), C.POINTER(C.c_int64)]),

@@ -163,6 +163,7 @@ int bta_ws_destroy(bta_ws_t ws) {
   if (ws->ev_out) hipEventDestroy(ws->ev_out);
   ws->w.trtri_plan.reset();
   ws->w.potri_plan.reset();
+  ws->w.update_plan.reset();
   hipFree(ws->w.winv);
   hipFree(ws->w.flags);
   if (ws->w.wfull) hipFree(ws->w.wfull);
@@ -176,6 +177,8 @@ int bta_potrf(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
   if (!D || (n > 1 && !L) || (a > 0 && (!B || !T))) return BTA_E_ARG;
   GraphKey key{1, {(uintptr_t)D, (uintptr_t)L, (uintptr_t)B, (uintptr_t)T}, {n, b, a}};
   ws->w.w_valid = false;
+  rc = bta::potrf_prepare(ws->w, D, L, B, T, n, b, a);
+  if (rc) return rc;
   return run_maybe_graph(ws, key, as_stream(stream), [&](hipStream_t st) {
     return bta::potrf_enqueue(ws->w, D, L, B, T, n, b, a, st);
   });
@@ -350,6 +353,19 @@ int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* st
 }
 
 double* bta_ws_logdet_ptr(bta_ws_t ws) { return ws ? ws->w.logdet : nullptr; }
+
+int bta_ws_set_option(bta_ws_t ws, int option, int value) {
+  if (!ws) return BTA_E_ARG;
+  if (value != 0 && value != 64 && value != 128) return BTA_E_ARG;
+  std::lock_guard<std::mutex> g(ws->mu);
+  if (option == BTA_OPT_TILE_UPDATE) ws->w.tile_update = value;
+  else if (option == BTA_OPT_TILE_GROUPED) ws->w.tile_grouped = value;
+  else return BTA_E_ARG;
+  // recorded graphs embed the old launch shapes
+  for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);
+  ws->graphs.clear();
+  return 0;
+}
 
 int bta_ws_set_timing(bta_ws_t ws, int on) {
   if (!ws) return BTA_E_ARG;

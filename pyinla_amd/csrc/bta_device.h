@@ -76,22 +76,32 @@ __device__ __forceinline__ void lds_load_tile_t(double* __restrict__ s, const do
 // multiplied.  The LDS layout is always s[i][k] (row i of op(X), column k).
 //   trans == 0: op(X) = X, X row-major (rows x K): s[i][k] = X[i][k]
 //   trans == 1: op(X) = X^T, X row-major (K x rows): s[i][k] = X[k][i]
+// Element-exact triangular masks for operand tiles (row index i of op(X),
+// K index k, both global): MASK_K_GT_I zeroes k > i, MASK_K_LT_I zeroes k < i.
+enum : int { MASK_NONE = 0, MASK_K_GT_I = 1, MASK_K_LT_I = 2 };
+
+__device__ __forceinline__ bool masked_out(int mask, int ig, int kg) {
+  return (mask == MASK_K_GT_I && kg > ig) || (mask == MASK_K_LT_I && kg < ig);
+}
+
 struct TilePrefetch {
   double r[16];
-  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw) {
+  // i0 / k0: global row / K index of the tile origin (used by the mask only)
+  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw,
+                                       int mask = MASK_NONE, int i0 = 0, int k0 = 0) {
     const int c = threadIdx.x & 63;
     const int r0 = threadIdx.x >> 6;
     if (trans == 0) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int i = r0 + 4 * j;
-        r[j] = (i < rows && c < kw) ? g[(int64_t)i * ld + c] : 0.0;
+        r[j] = (i < rows && c < kw && !masked_out(mask, i0 + i, k0 + c)) ? g[(int64_t)i * ld + c] : 0.0;
       }
     } else {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int k = r0 + 4 * j;
-        r[j] = (k < kw && c < rows) ? g[(int64_t)k * ld + c] : 0.0;
+        r[j] = (k < kw && c < rows && !masked_out(mask, i0 + c, k0 + k)) ? g[(int64_t)k * ld + c] : 0.0;
       }
     }
   }

@@ -106,10 +106,12 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __r
   auto fetch = [&]() {
     const GemmSeg& sg = g.seg[seg_id];
     const int kw = min(64, kmax - k0);
-    if (sg.ta == 0) pa.load(sg.A + (int64_t)(tr * 64) * sg.lda + k0, sg.lda, 0, rows, kw);
-    else pa.load(sg.A + (int64_t)k0 * sg.lda + tr * 64, sg.lda, 1, rows, kw);
-    if (sg.tb == 1) pb.load(sg.B + (int64_t)(tc * 64) * sg.ldb + k0, sg.ldb, 0, cols, kw);
-    else pb.load(sg.B + (int64_t)k0 * sg.ldb + tc * 64, sg.ldb, 1, cols, kw);
+    const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
+    const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
+    if (sg.ta == 0) pa.load(sg.A + (int64_t)(tr * 64) * sg.lda + k0, sg.lda, 0, rows, kw, ma, tr * 64, k0);
+    else pa.load(sg.A + (int64_t)k0 * sg.lda + tr * 64, sg.lda, 1, rows, kw, ma, tr * 64, k0);
+    if (sg.tb == 1) pb.load(sg.B + (int64_t)(tc * 64) * sg.ldb + k0, sg.ldb, 0, cols, kw, mb, tc * 64, k0);
+    else pb.load(sg.B + (int64_t)k0 * sg.ldb + tc * 64, sg.ldb, 1, cols, kw, mb, tc * 64, k0);
   };
   bool have = (g.nseg > 0) && next_chunk(true);
   if (have) fetch();
@@ -162,16 +164,17 @@ struct BigOp {
   double r[BIG_TS * BIG_KC / BTA_THREADS];  // 8
   // trans == 0: X row-major rows x K -> element (i, k) = X[i][k]
   // trans == 1: X row-major K x rows -> element (i, k) = X[k][i]
-  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw) {
+  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw,
+                                       int mask, int i0, int k0) {
 #pragma unroll
     for (int p = 0; p < 8; ++p) {
       const int e = threadIdx.x + BTA_THREADS * p;
       if (trans == 0) {
         const int i = e >> 4, k = e & 15;
-        r[p] = (i < rows && k < kw) ? g[(int64_t)i * ld + k] : 0.0;
+        r[p] = (i < rows && k < kw && !masked_out(mask, i0 + i, k0 + k)) ? g[(int64_t)i * ld + k] : 0.0;
       } else {
         const int k = e >> 7, i = e & 127;
-        r[p] = (i < rows && k < kw) ? g[(int64_t)k * ld + i] : 0.0;
+        r[p] = (i < rows && k < kw && !masked_out(mask, i0 + i, k0 + k)) ? g[(int64_t)k * ld + i] : 0.0;
       }
     }
   }
@@ -250,10 +253,12 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped_big(const GemmJob*
   auto fetch = [&]() {
     const GemmSeg& sg = g.seg[seg_id];
     const int kw = min(BIG_KC, kmax - k0);
-    if (sg.ta == 0) pa.load(sg.A + (int64_t)(tr * BIG_TS) * sg.lda + k0, sg.lda, 0, rows, kw);
-    else pa.load(sg.A + (int64_t)k0 * sg.lda + tr * BIG_TS, sg.lda, 1, rows, kw);
-    if (sg.tb == 1) pb.load(sg.B + (int64_t)(tc * BIG_TS) * sg.ldb + k0, sg.ldb, 0, cols, kw);
-    else pb.load(sg.B + (int64_t)k0 * sg.ldb + tc * BIG_TS, sg.ldb, 1, cols, kw);
+    const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
+    const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
+    if (sg.ta == 0) pa.load(sg.A + (int64_t)(tr * BIG_TS) * sg.lda + k0, sg.lda, 0, rows, kw, ma, tr * BIG_TS, k0);
+    else pa.load(sg.A + (int64_t)k0 * sg.lda + tr * BIG_TS, sg.lda, 1, rows, kw, ma, tr * BIG_TS, k0);
+    if (sg.tb == 1) pb.load(sg.B + (int64_t)(tc * BIG_TS) * sg.ldb + k0, sg.ldb, 0, cols, kw, mb, tc * BIG_TS, k0);
+    else pb.load(sg.B + (int64_t)k0 * sg.ldb + tc * BIG_TS, sg.ldb, 1, cols, kw, mb, tc * BIG_TS, k0);
   };
   bool have = (g.nseg > 0) && next_chunk(true);
   if (have) fetch();

@@ -53,12 +53,20 @@ struct Workspace {
   bool w_valid = false;            // W matches the factor currently held
   const double* w_of = nullptr;    // D pointer W was computed from
   // cached launch plans, keyed by the block pointers they were built for
-  std::unique_ptr<DevPlan> trtri_plan, potri_plan;
+  std::unique_ptr<DevPlan> trtri_plan, potri_plan, update_plan;
+  const void* update_key[4] = {nullptr, nullptr, nullptr, nullptr};
+  int update_a = -1;
   const void* trtri_key = nullptr;
+  int trtri_tile = -1, potri_tile = -1;
   const void* potri_key[4] = {nullptr, nullptr, nullptr, nullptr};
   int potri_a = -1;
   TimingLog* tlog = nullptr;  // non-null: bracket launches with events
+  // Output-tile edge per GEMM class: 0 = heuristic (pick_tile), 64 or 128.
+  int tile_update = 0;   // block updates of the factorization
+  int tile_grouped = 0;  // block inverses and selected inversion
 };
+
+inline int choose_tile(int pref, int m, int n) { return pref ? pref : pick_tile(m, n); }
 
 // RAII bracket: records a start event now and the stop event at scope exit.
 struct TimedScope {
@@ -107,6 +115,9 @@ struct BatchPlan {
   int enqueue(hipStream_t st) const;
 };
 
+// Builds the grouped block-update plan when the blocks are large enough for
+// 128x128 tiles (else the by-value update kernel is used); host only.
+int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a);
 int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
                   hipStream_t st);
 // Host-side preparation (allocations, job tables); never called during capture.

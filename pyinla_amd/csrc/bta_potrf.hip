@@ -420,8 +420,53 @@ static void launch_gemm_jobs(UpdJobs& jobs, hipStream_t st) {
   if (total > 0) hipLaunchKernelGGL(k_gemm_nt_sub, dim3(total), dim3(BTA_THREADS), 0, st, jobs);
 }
 
+int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a) {
+  if (choose_tile(ws.tile_update, b, b) != 128) {
+    ws.update_plan.reset();
+    return 0;
+  }
+  const void* key[4] = {D, L, B, T};
+  bool same = ws.update_plan && ws.update_a == a;
+  for (int q = 0; q < 4 && same; ++q) same = (ws.update_key[q] == key[q]);
+  if (same) return 0;
+  std::vector<GemmLaunch> ls;
+  for (int i = 0; i < n; ++i) {
+    GemmLaunch g(128);
+    double* Li = (i < n - 1) ? L + (int64_t)i * b * b : nullptr;
+    double* Bi = (a > 0) ? B + (int64_t)i * a * b : nullptr;
+    if (i < n - 1) {
+      GemmJob j = job(D + (int64_t)(i + 1) * b * b, b, b, b, 1.0, 1, 0);
+      add_seg(j, seg(Li, b, 0, Li, b, 1, b, -1.0));
+      g.add(j);
+      if (a > 0) {
+        GemmJob jb = job(B + (int64_t)(i + 1) * a * b, a, b, b, 1.0);
+        add_seg(jb, seg(Bi, b, 0, Li, b, 1, b, -1.0));
+        g.add(jb);
+      }
+    }
+    if (a > 0) {
+      GemmJob jt = job(T, a, a, a, 1.0, 1, 0);
+      add_seg(jt, seg(Bi, b, 0, Bi, b, 1, b, -1.0));
+      g.add(jt);
+    }
+    ls.push_back(std::move(g));
+  }
+  ws.update_plan.reset(new DevPlan());
+  const int rc = ws.update_plan->build(ls);
+  if (rc) {
+    ws.update_plan.reset();
+    return rc;
+  }
+  for (int q = 0; q < 4; ++q) ws.update_key[q] = key[q];
+  ws.update_a = a;
+  return 0;
+}
+
 int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
                   hipStream_t st) {
+  const void* key[4] = {D, L, B, T};
+  bool use_plan = ws.update_plan && ws.update_a == a;
+  for (int q = 0; q < 4 && use_plan; ++q) use_plan = (ws.update_key[q] == key[q]);
   const int nT = (b + 63) / 64;
   const int nTa = (a + 63) / 64;
   hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, st, ws.epoch, ws.info, ws.err);
@@ -460,7 +505,8 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     }
     if (a > 0) jobs.job[jobs.njobs++] = UpdJob{T, p.B, p.B, a, a, b, a, b, b, 1, 0, 0};
     TimedScope ts(ws, KC_UPDATE, st);
-    launch_gemm_jobs(jobs, st);
+    if (use_plan) ws.update_plan->launch(i, st);
+    else launch_gemm_jobs(jobs, st);
   }
   int npart = n * nT;
   if (a > 0) {

@@ -46,11 +46,11 @@ int trtri_prepare(Workspace& ws, const double* D, int n, int b) {
     hipError_t e = hipMalloc(&ws.wfull, (size_t)n * b * b * sizeof(double));
     if (e != hipSuccess) return (int)e;
   }
-  if (ws.trtri_plan && ws.trtri_key == D) return 0;
+  if (ws.trtri_plan && ws.trtri_key == D && ws.trtri_tile == ws.tile_grouped) return 0;
   const int nT = (b + 63) / 64;
   std::vector<GemmLaunch> ls;
   for (int s = 1; s < nT; s *= 2) {
-    const int ts = pick_tile(s * 64, s * 64);
+    const int ts = choose_tile(ws.tile_grouped, s * 64, s * 64);
     GemmLaunch la(ts), lb(ts);
     for (int t0 = 0; t0 + s < nT; t0 += 2 * s) {
       const int r1 = t0 * 64, r2 = (t0 + s) * 64;
@@ -79,6 +79,7 @@ int trtri_prepare(Workspace& ws, const double* D, int n, int b) {
     return rc;
   }
   ws.trtri_key = D;
+  ws.trtri_tile = ws.tile_grouped;
   return 0;
 }
 
@@ -97,13 +98,13 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
   int rc = trtri_prepare(ws, D, n, b);
   if (rc) return rc;
   const void* key[4] = {D, L, B, T};
-  bool same = ws.potri_plan && ws.potri_a == a;
+  bool same = ws.potri_plan && ws.potri_a == a && ws.potri_tile == ws.tile_grouped;
   for (int q = 0; q < 4 && same; ++q) same = (ws.potri_key[q] == key[q]);
   if (same) return 0;
   double* G = ws.work;                      // b x b
   double* Ga = ws.work + (size_t)b * b;     // a x b
   std::vector<GemmLaunch> ls;
-  const int ts = pick_tile(b, b);
+  const int ts = choose_tile(ws.tile_grouped, b, b);
   if (a > 0) {  // Sigma_T = Tinv^T Tinv
     GemmLaunch l0;
     GemmJob j = job(T, a, a, a, 0.0);
@@ -157,6 +158,7 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
   }
   for (int q = 0; q < 4; ++q) ws.potri_key[q] = key[q];
   ws.potri_a = a;
+  ws.potri_tile = ws.tile_grouped;
   return 0;
 }
 

@@ -35,6 +35,7 @@ __device__ __forceinline__ f64x4 mma16(const double* __restrict__ A, int ra, int
                                        int kb, int cb, int K, double alpha, f64x4 acc) {
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 1
   for (int k = 0; k < K; k += 4) {
     const double a = alpha * A[(ra + li) * BTA_LDS_LD + ka + k + lk];
     const double b = BT ? B[(cb + li) * BTA_LDS_LD + kb + k + lk] : B[(kb + k + lk) * BTA_LDS_LD + cb + li];
@@ -62,6 +63,9 @@ __device__ __forceinline__ void store16(double* __restrict__ C, int r0, int c0, 
 // On exit: s holds L (lower, zero strict upper, identity outside mv),
 //          w holds L^-1 (lower), both with stride BTA_LDS_LD.
 // scratch: LDS, >= 64 + 64 doubles (pivots and their logs).
+// Code size matters as much as instruction count here: the workgroup that runs
+// this is on the critical path of every panel launch and starts with a cold
+// instruction cache, so only the 16-step elimination is unrolled.
 __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, double* __restrict__ w,
                                                          double* __restrict__ scratch, int mv) {
   const int lane = threadIdx.x & 63;
@@ -78,6 +82,7 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
   }
   __syncthreads();
 
+#pragma unroll 1
   for (int blk = 0; blk < 4; ++blk) {
     const int c0 = 16 * blk;
     // (a) 16x16 diagonal block: factor + inverse on wave 0
@@ -90,30 +95,28 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
         e[c] = (c == i) ? 1.0 : 0.0;
       }
       double myd = 1.0;
-#pragma unroll
+      // Runtime loops with wave-uniform indices: the register arrays are
+      // addressed indirectly (no scratch) and the code stays a few hundred
+      // instructions long.
+#pragma unroll 1
       for (int j = 0; j < 16; ++j) {
-        const double d = readlane_f64(a[j], j);
+        const double ajj = a[j];
+        const double d = readlane_f64(ajj, j);
         if (lane == 0) {
           piv[c0 + j] = d;
           if (c0 + j < mv && fail < 0 && !(d > 0.0)) fail = c0 + j;
         }
         if (i == j) myd = d;
         const double sinv = 1.0 / d;
-        const double mi = a[j] * sinv;  // multiplier of row i (used for i > j)
-#pragma unroll
-        for (int l = j + 1; l < 16; ++l) {
-          const double alj = readlane_f64(a[j], l);  // A[l][j]
-          if (i > j) a[l] = fma(-mi, alj, a[l]);
-        }
-#pragma unroll
-        for (int c = 0; c <= j; ++c) {
-          const double ejc = readlane_f64(e[c], j);  // E[j][c]
-          if (i > j) e[c] = fma(-mi, ejc, e[c]);
-        }
+        const double mi = (i > j) ? ajj * sinv : 0.0;  // multiplier of row i
+#pragma unroll 1
+        for (int l = j + 1; l < 16; ++l) a[l] = fma(-mi, readlane_f64(ajj, l), a[l]);
+#pragma unroll 1
+        for (int c = 0; c <= j; ++c) e[c] = fma(-mi, readlane_f64(e[c], j), e[c]);
       }
       const double rsi = 1.0 / sqrt(myd);
       if (lane < 16) {
-#pragma unroll
+#pragma unroll 4
         for (int c = 0; c < 16; ++c) {
           const double dc = piv[c0 + c];
           double lv = 0.0;
@@ -139,6 +142,7 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
     // (c) trailing lower tiles: A[ri][ci] -= X[ri] X[ci]^T, blk < ci <= ri <= 3
     if (nbelow > 0) {
       const int ntr = nbelow * (nbelow + 1) / 2;
+#pragma unroll 1
       for (int q = wv; q < ntr; q += 4) {
         int rr = 0, cc = q;
         while (cc > rr) {
@@ -160,12 +164,14 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
     if (j > i) s[i * BTA_LDS_LD + j] = 0.0;
   }
   // inverse, block row by block row: W_rc = -W_rr sum_{c<=k<r} L_rk W_kc
+#pragma unroll 1
   for (int r = 1; r < 4; ++r) {
     __syncthreads();
     // T_rc = sum_{k=c}^{r-1} L_rk W_kc, parked in the (still empty) block (c, r) of w
     if (wv < r) {
       const int c = wv;
       f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
       for (int k = c; k < r; ++k) acc = mma16<false>(s, 16 * r, 16 * k, w, 16 * k, 16 * c, 16, 1.0, acc);
       store16(w, 16 * c, 16 * r, acc);
     }

@@ -120,3 +120,23 @@ def test_batched_factorization_mixed_shapes(cuda):
                 assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < RTOL
             if B is not None:
                 assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
+
+
+def test_autotuner_keeps_results_exact(cuda):
+    """The autotuner times both tile edges on the solver's matrix, restores the
+    blocks, and the factorization / selected inversion with the chosen tiles
+    still match the oracle."""
+    from pyinla_amd import autotune
+    kf = known_factor_bta(3, 600, 5, seed=77)
+    s = SerinvSolver(None, 600, 3, 5)
+    s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B, kf.Q_T)
+    choice = autotune.autotune(s, "bta", reps=1)
+    assert choice["update"] in (64, 128) and choice["grouped"] in (64, 128)
+    assert _rel(s.A_diagonal_blocks.cpu().numpy(), kf.Q_D) < 1e-15  # restored
+    s.factorize("bta")
+    assert _rel(s.A_diagonal_blocks.cpu().numpy(), kf.Lf_D) < RTOL
+    s.selected_inversion("bta")
+    D, L, B, T = kf.Lf_D.copy(), kf.Lf_L.copy(), kf.Lf_B.copy(), kf.Lf_T.copy()
+    ob.pobtasi(D, L, B, T)
+    assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
+    assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
