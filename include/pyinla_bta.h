@@ -101,7 +101,26 @@ double* bta_ws_logdet_ptr(bta_ws_t ws);
  * (pyinla_amd/autotune.py); recorded graphs are dropped. */
 #define BTA_OPT_TILE_UPDATE 1
 #define BTA_OPT_TILE_GROUPED 2
+/* Panel factorization of bta_potrf: 0 = automatic, 1 = one launch per 64-wide
+ * tile column, 2 = one persistent dataflow launch per block (a chain
+ * workgroup on the diagonal plus helper workgroups running a dependency-
+ * levelled task list).  BTA_OPT_FLOW_HELPERS sets the helper count (0 =
+ * automatic: half the device's resident capacity, less one) and
+ * BTA_OPT_FLOW_DELTA how many tile columns far-from-diagonal updates may be
+ * deferred (default 2). */
+#define BTA_OPT_PANEL 3
+#define BTA_OPT_FLOW_HELPERS 4
+#define BTA_OPT_FLOW_DELTA 5
 int bta_ws_set_option(bta_ws_t ws, int option, int value);
+
+/* Host-only diagnostic (no reference counterpart): the helper task lists of
+ * the persistent panel launch for a panel of nT tile columns, nTL L tile rows
+ * and nTa (0/1) arrow tile rows.  Task words are op << 30 | row << 20 |
+ * col << 10 | k (op 0 = update with column k, 1 = solve, 2 = clear);
+ * off[0..helpers] delimit the helpers' lists.  With tasks == off == NULL only
+ * *ntasks is returned.  Used by the CPU schedule simulator in the tests. */
+int bta_flow_schedule(int nT, int nTL, int nTa, int helpers, int delta, uint32_t* tasks, int64_t cap, int* off,
+                      int64_t* ntasks);
 
 /* Optional event timing (no reference counterpart; the reference times whole
  * calls with perf_counter, structured_solver.py:80-100).  While on, graphs are

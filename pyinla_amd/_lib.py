@@ -38,6 +38,7 @@ SIGNATURES = {
     "bta_gather_coo": (_I, [_P, _P, _P, _P, _P, _P, C.c_int64, _P]),
     "bta_bilinear_coo": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int64, _I, _P, _I, _P, _P]),
     "bta_kron_contract": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P]),
+    "bta_flow_schedule": (_I, [_I, _I, _I, _I, _I, _P, C.c_int64, _P, C.POINTER(C.c_int64)]),
 }
 
 BTA_E = {-1: "invalid argument", -2: "shape does not match the workspace", -3: "in-kernel wait timed out"}

@@ -14,7 +14,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libpyinla_bta.so"
-SOURCES = ["bta_capi.hip", "bta_gemm.hip", "bta_potrf.hip", "bta_potrs.hip", "bta_potri.hip", "bta_assemble.hip"]
+SOURCES = ["bta_capi.hip", "bta_gemm.hip", "bta_potrf.hip", "bta_flow.hip", "bta_potrs.hip", "bta_potri.hip",
+           "bta_assemble.hip"]
 ARCH = os.environ.get("PYINLA_AMD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

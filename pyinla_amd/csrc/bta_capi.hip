@@ -354,12 +354,34 @@ int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* st
 
 double* bta_ws_logdet_ptr(bta_ws_t ws) { return ws ? ws->w.logdet : nullptr; }
 
+int bta_flow_schedule(int nT, int nTL, int nTa, int helpers, int delta, uint32_t* tasks, int64_t cap, int* off,
+                      int64_t* ntasks) {
+  if (nT < 1 || nT > 1023 || nTL < 0 || nTa < 0 || nTa > 1 || helpers < 1 || helpers > 1023 || delta < 0 ||
+      2 * nT + nTa > 1023)
+    return BTA_E_ARG;
+  bta::FlowSchedule sc;
+  bta::flow_build_schedule(bta::FlowShape{nT, nTL, nTa}, helpers, delta, sc);
+  if (ntasks) *ntasks = (int64_t)sc.tasks.size();
+  if (!tasks && !off) return 0;
+  if (!tasks || !off || cap < (int64_t)sc.tasks.size()) return BTA_E_ARG;
+  for (size_t q = 0; q < sc.tasks.size(); ++q) tasks[q] = sc.tasks[q];
+  for (int h = 0; h <= helpers; ++h) off[h] = sc.off[h];
+  return 0;
+}
+
 int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   if (!ws) return BTA_E_ARG;
-  if (value != 0 && value != 64 && value != 128) return BTA_E_ARG;
+  const bool tile_opt = (option == BTA_OPT_TILE_UPDATE || option == BTA_OPT_TILE_GROUPED);
+  if (tile_opt && value != 0 && value != 64 && value != 128) return BTA_E_ARG;
+  if (option == BTA_OPT_PANEL && (value < 0 || value > 2)) return BTA_E_ARG;
+  if (option == BTA_OPT_FLOW_HELPERS && (value < 0 || value > 1023)) return BTA_E_ARG;
+  if (option == BTA_OPT_FLOW_DELTA && (value < 0 || value > 64)) return BTA_E_ARG;
   std::lock_guard<std::mutex> g(ws->mu);
   if (option == BTA_OPT_TILE_UPDATE) ws->w.tile_update = value;
   else if (option == BTA_OPT_TILE_GROUPED) ws->w.tile_grouped = value;
+  else if (option == BTA_OPT_PANEL) ws->w.panel_mode = value;
+  else if (option == BTA_OPT_FLOW_HELPERS) ws->w.flow_helpers = value;
+  else if (option == BTA_OPT_FLOW_DELTA) ws->w.flow_delta = value;
   else return BTA_E_ARG;
   // recorded graphs embed the old launch shapes
   for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);

@@ -1,0 +1,77 @@
+// Persistent dataflow factorization of one BTA block column (host + device).
+//
+// The stacked panel P_i = [D_i; L_i; B_i] of block i is split into 64x64
+// tiles (ρ, c): stacked tile row ρ (D rows 0..nT-1, L rows nT..nT+nTL-1, the
+// arrow row after them), tile column c.  Its right-looking tile Cholesky is
+// run by ONE launch of 1 + P workgroups that stay resident:
+//
+//   * workgroup 0, the chain, walks the diagonal: with L_kk and inv(L_kk) of
+//     step k in LDS it solves the sub-diagonal tile (k+1, k), applies that
+//     tile's rank-64 product to the diagonal tile (k+1, k+1) and factors it;
+//   * workgroups 1..P, the helpers, run every other tile task -- the solve
+//     X(ρ, k) = C(ρ, k) inv(L_kk)^T and the rank-64 updates
+//     C(ρ, c) -= X(ρ, k) X(c, k)^T -- from a task list built on the host.
+//
+// Every task has a level: Winv_k (chain) 3k, solves of column k 3k+1, updates
+// with column j at 3 min(j + δ, cap) + 2, where the cap keeps a tile's updates
+// before its own solve (cap = c - 1) or before the chain takes it over
+// (cap = c - 2 on the diagonal).  Each helper runs its tasks in increasing
+// level and every task depends only on lower levels, so, with all workgroups
+// resident, the lowest unfinished task can always run: no ordering can
+// deadlock.  δ > 0 defers updates of tiles far from the diagonal, which
+// evens out the per-column load of the helpers.
+//
+// Memory protocol (the L2 caches of the XCDs are not coherent): every address
+// is stored to by one workgroup only.  Helpers accumulate their tiles in place
+// with ordinary loads and stores; a finished tile is also written once, with
+// system-scope (write-through) stores, to an exchange buffer that readers load
+// with system-scope loads, and a flag carrying the call tag is stored after
+// the data has completed.  The diagonal and sub-diagonal tiles the chain takes
+// over are accumulated by their helpers in a separate hand-off buffer so that
+// only the chain ever stores to their place in the factor.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace bta {
+
+enum : int { FLOW_UPD = 0, FLOW_TRSM = 1, FLOW_ZERO = 2 };
+
+// Task word: op (2 bits) | ρ (10) | c (10) | k (10).
+__host__ __device__ inline uint32_t flow_encode(int op, int rho, int c, int k) {
+  return ((uint32_t)op << 30) | ((uint32_t)rho << 20) | ((uint32_t)c << 10) | (uint32_t)k;
+}
+__host__ __device__ inline void flow_decode(uint32_t w, int& op, int& rho, int& c, int& k) {
+  op = (int)(w >> 30);
+  rho = (int)((w >> 20) & 1023u);
+  c = (int)((w >> 10) & 1023u);
+  k = (int)(w & 1023u);
+}
+
+// Shape of a panel: nT tile columns (and D tile rows), nTL L tile rows (0 for
+// the last block), nTa arrow tile rows (0 or 1).
+struct FlowShape {
+  int nT, nTL, nTa;
+  int rows() const { return nT + nTL + nTa; }
+};
+
+// Tiles the chain takes over: the sub-diagonal (k+1, k) and diagonal
+// (k+1, k+1) tiles for k + 1 < nT.  Their helper updates go through the
+// hand-off buffer.
+__host__ __device__ inline bool flow_is_handoff(int nT, int rho, int c) {
+  return rho < nT && ((rho == c && c >= 1) || (rho == c + 1));
+}
+
+// Helper task lists for one panel shape.  tasks[off[h] .. off[h+1]) belong to
+// helper h (0-based), in execution order.
+struct FlowSchedule {
+  std::vector<uint32_t> tasks;
+  std::vector<int> off;
+  int helpers = 0;
+};
+
+int flow_owner(const FlowShape& s, int rho, int c, int helpers);
+void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedule& out);
+
+}  // namespace bta

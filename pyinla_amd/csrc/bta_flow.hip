@@ -1,0 +1,444 @@
+// Persistent dataflow panel factorization (see bta_flow.h for the scheme).
+//
+// Replaces the nT launches per block of the column-by-column panel with one
+// launch of 1 + P resident workgroups: the chain (workgroup 0) runs the
+// diagonal Cholesky + inverse, TRSM of the sub-diagonal tile and the rank-64
+// update of the next diagonal tile back to back, while P helpers work through
+// their task lists.  Same algorithm and outputs as the per-column path
+// (serinv pobtaf's block loop, serinv/algs/pobtaf.py:108-170): L_ii in D_i,
+// L_{i+1,i} in L_i, L_{a,i} in B_i, inverse of every diagonal tile in winv and
+// per-tile log-pivot sums in logpart.
+#include <algorithm>
+
+#include "bta_internal.h"
+#include "bta_flow.h"
+#include "bta_tile_chol.h"
+
+namespace bta {
+
+// ---------------------------------------------------------------- schedule
+
+// Tiles are dealt round-robin in column-major order (column c: D rows c..nT-1,
+// then the L and arrow rows), so every helper owns tiles spread evenly over
+// the columns and its share of each column's updates stays balanced as the
+// factorization advances.  Cleared upper tiles of D go by row-major index.
+int flow_owner(const FlowShape& s, int rho, int c, int helpers) {
+  const int nT = s.nT, below = s.nTL + s.nTa;
+  if (rho < nT && rho < c) return (int)(((long long)rho * nT + c) % helpers);
+  // tiles in columns 0..c-1: sum (nT - c' + below)
+  long long t = (long long)c * (nT + below) - (long long)c * (c - 1) / 2;
+  t += (rho < nT) ? rho - c : (nT - c) + (rho - nT);
+  return (int)(t % helpers);
+}
+
+void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedule& out) {
+  struct Item {
+    unsigned long long key;
+    uint32_t word;
+    int h;
+  };
+  std::vector<Item> v;
+  const int nT = s.nT, R = s.rows();
+  auto push = [&](int level, int prio, int op, int rho, int c, int k) {
+    const unsigned long long key = ((unsigned long long)level << 42) | ((unsigned long long)prio << 31) |
+                                   ((unsigned long long)c << 21) | ((unsigned long long)rho << 10) |
+                                   (unsigned long long)k;
+    v.push_back({key, flow_encode(op, rho, c, k), flow_owner(s, rho, c, helpers)});
+  };
+  for (int rho = 0; rho < R; ++rho) {
+    for (int c = 0; c < nT; ++c) {
+      if (rho < nT && rho < c) {  // strict upper tile of D: cleared
+        push(0, 0, FLOW_ZERO, rho, c, 0);
+        continue;
+      }
+      const bool diag = (rho == c);
+      const bool hand = flow_is_handoff(nT, rho, c);
+      const int cap = diag ? c - 2 : c - 1;
+      for (int j = 0; j < c; ++j) {
+        if (diag && j == c - 1) continue;  // applied by the chain
+        const int round = hand ? j : std::min(j + delta, cap);
+        const int prio = hand ? 0 : 1 + (c - round);
+        push(3 * round + 2, prio, FLOW_UPD, rho, c, j);
+      }
+      if (!diag && !(rho < nT && rho == c + 1)) push(3 * c + 1, rho - c, FLOW_TRSM, rho, c, c);
+    }
+  }
+  std::stable_sort(v.begin(), v.end(), [](const Item& x, const Item& y) {
+    return x.h != y.h ? x.h < y.h : x.key < y.key;
+  });
+  out.helpers = helpers;
+  out.tasks.resize(v.size());
+  out.off.assign(helpers + 1, 0);
+  for (size_t q = 0; q < v.size(); ++q) {
+    out.tasks[q] = v[q].word;
+    out.off[v[q].h + 1] += 1;
+  }
+  for (int h = 0; h < helpers; ++h) out.off[h + 1] += out.off[h];
+}
+
+// ------------------------------------------------------------------ device
+
+struct FlowArgs {
+  double* D;                 // D_i (b x b)
+  double* L;                 // L_i or nullptr
+  double* B;                 // B_i (a x b) or nullptr
+  double* winv;              // nT tiles: inverse of each diagonal tile of D_i
+  double* xbuf;              // R * nT tiles: exchange copy of every finished tile
+  double* hacc;              // 2 * nT tiles: hand-off accumulators (sub-diagonal, diagonal)
+  double* logpart;           // nT partial log sums
+  unsigned long long* flags; // R*nT finished tiles, nT tile inverses, 2*nT hand-offs
+  const uint32_t* tasks;
+  const int* off;            // helpers + 1
+  const unsigned long long* epoch;
+  unsigned long long* info;
+  int* err;
+  long long row0;
+  unsigned long long spin_ticks;  // bound on one wait, in s_memrealtime ticks
+  int b, a, nT, nTL, nTa;
+  int blk;                   // block index + 1 (part of the flag tag)
+};
+
+__device__ __forceinline__ double* flow_tile(const FlowArgs& p, int rho, int c, int& rows, int& cols) {
+  cols = min(64, p.b - 64 * c);
+  if (rho < p.nT) {
+    rows = min(64, p.b - 64 * rho);
+    return p.D + (int64_t)(64 * rho) * p.b + 64 * c;
+  }
+  rho -= p.nT;
+  if (rho < p.nTL) {
+    rows = min(64, p.b - 64 * rho);
+    return p.L + (int64_t)(64 * rho) * p.b + 64 * c;
+  }
+  rho -= p.nTL;
+  rows = min(64, p.a - 64 * rho);
+  return p.B + (int64_t)(64 * rho) * p.b + 64 * c;
+}
+
+__device__ __forceinline__ int flow_rows(const FlowArgs& p) { return p.nT + p.nTL + p.nTa; }
+__device__ __forceinline__ double* xtile(const FlowArgs& p, int rho, int c) {
+  return p.xbuf + ((int64_t)rho * p.nT + c) * 4096;
+}
+__device__ __forceinline__ unsigned long long* xflag(const FlowArgs& p, int rho, int c) {
+  return p.flags + (int64_t)rho * p.nT + c;
+}
+__device__ __forceinline__ unsigned long long* wflag(const FlowArgs& p, int k) {
+  return p.flags + (int64_t)flow_rows(p) * p.nT + k;
+}
+__device__ __forceinline__ unsigned long long* hflag(const FlowArgs& p, int k, int s) {
+  return p.flags + (int64_t)flow_rows(p) * p.nT + p.nT + 2 * k + s;
+}
+__device__ __forceinline__ double* hsub(const FlowArgs& p, int c) { return p.hacc + (int64_t)c * 4096; }
+__device__ __forceinline__ double* hdiag(const FlowArgs& p, int c) {
+  return p.hacc + ((int64_t)p.nT + c) * 4096;
+}
+
+// Bounded wait for *flag == tag.  Gives up (returns false and raises err) after
+// spin_ticks, or at once when another workgroup has raised err, so that every
+// workgroup still reaches the end of its list.
+__device__ __forceinline__ bool flow_wait(const FlowArgs& p, unsigned long long* flag, unsigned long long tag,
+                                          int* ok_sh) {
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != tag) {
+      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+      while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != tag) {
+        if (__hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) { ok = 0; break; }
+        if (__builtin_amdgcn_s_memrealtime() - t0 > p.spin_ticks) {
+          atomicOr(p.err, 1);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    *ok_sh = ok;
+  }
+  __syncthreads();
+  const int ok = *ok_sh;
+  __syncthreads();
+  return ok != 0;
+}
+
+// 64x64 contiguous tile <-> registers / LDS with system-scope accesses.
+__device__ __forceinline__ void acc_load_sys(TileAcc& t, const double* __restrict__ g) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) t.c[x][y][r] = ld_sys(&g[acc_row(x, r) * 64 + acc_col(y)]);
+}
+
+__device__ __forceinline__ void acc_store_sys(const TileAcc& t, double* __restrict__ g) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) st_sys(&g[acc_row(x, r) * 64 + acc_col(y)], t.c[x][y][r]);
+}
+
+__device__ __forceinline__ void lds_load_sys(double* __restrict__ s, const double* __restrict__ g) {
+  const int c = threadIdx.x & 63;
+#pragma unroll 4
+  for (int r = threadIdx.x >> 6; r < 64; r += 4) s[r * BTA_LDS_LD + c] = ld_sys(&g[r * 64 + c]);
+}
+
+// Stores, after the caller's st_sys data stores, the flag once they completed.
+__device__ __forceinline__ void flow_publish(unsigned long long* flag, unsigned long long tag) {
+  flag_publish(flag, tag);
+}
+
+// X = C inv(L_kk)^T from C in sC and inv(L_kk) (lower) in sW.
+__device__ __forceinline__ void flow_trsm(TileAcc& x, const double* sC, const double* sW) {
+  acc_zero(x);
+  const int kend = ((threadIdx.x >> 6) & 1) * 32 + 32;
+  mma_nt<false>(x, sC, sW, 0, kend);
+}
+
+// Chain: publish the factored diagonal tile k held in sA (L_kk) / sB (inverse).
+__device__ __forceinline__ void chain_publish_diag(const FlowArgs& p, int k, const TileCholResult& res,
+                                                   const double* sA, const double* sB, unsigned long long tag) {
+  int rows, cols;
+  double* g = flow_tile(p, k, k, rows, cols);
+  double* w = p.winv + (int64_t)k * 4096;
+  const int c = threadIdx.x & 63;
+  for (int r = threadIdx.x >> 6; r < 64; r += 4) {
+    if (r < rows && c < cols) g[(int64_t)r * p.b + c] = sA[r * BTA_LDS_LD + c];
+    st_sys(&w[r * 64 + c], sB[r * BTA_LDS_LD + c]);
+  }
+  if (threadIdx.x == 0) {
+    p.logpart[k] = res.logsum;
+    if (res.fail >= 0) atomicMin(p.info, (unsigned long long)(p.row0 + (long long)k * 64 + res.fail));
+  }
+  flow_publish(wflag(p, k), tag);
+}
+
+__device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA, double* sB, double* sScr,
+                           int* ok_sh) {
+  const int nT = p.nT;
+  int rows, cols;
+  TileAcc acc;
+  // F(0): the first diagonal tile has no in-panel updates
+  {
+    const double* g = flow_tile(p, 0, 0, rows, cols);
+    acc_load(acc, g, p.b, rows, cols);
+    acc_to_lds(acc, sA);
+    __syncthreads();
+    const TileCholResult res = tile_chol_inv(sA, sB, sScr, rows);
+    chain_publish_diag(p, 0, res, sA, sB, tag);
+  }
+  for (int k = 0; k + 1 < nT; ++k) {
+    // X(k+1, k) = C(k+1, k) inv(L_kk)^T ; C updated by its helper through column k-1
+    if (k == 0) {
+      const double* g = flow_tile(p, 1, 0, rows, cols);
+      acc_load(acc, g, p.b, rows, cols);
+    } else {
+      if (!flow_wait(p, hflag(p, k, 0), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 2);
+      acc_load_sys(acc, hsub(p, k));
+    }
+    acc_to_lds(acc, sA);
+    __syncthreads();
+    TileAcc x;
+    flow_trsm(x, sA, sB);
+    __syncthreads();
+    {
+      double* g = flow_tile(p, k + 1, k, rows, cols);
+      acc_store(x, g, p.b, rows, cols);
+      acc_store_sys(x, xtile(p, k + 1, k));
+      acc_to_lds(x, sA);
+    }
+    flow_publish(xflag(p, k + 1, k), tag);
+    // C(k+1, k+1) -= X X^T, then factor it
+    if (k == 0) {
+      const double* g = flow_tile(p, 1, 1, rows, cols);
+      acc_load(acc, g, p.b, rows, cols);
+    } else {
+      if (!flow_wait(p, hflag(p, k, 1), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 2);
+      acc_load_sys(acc, hdiag(p, k + 1));
+    }
+    mma_nt<true>(acc, sA, sA, 0, 64);
+    __syncthreads();
+    acc_to_lds(acc, sA);
+    __syncthreads();
+    const int mv = min(64, p.b - 64 * (k + 1));
+    const TileCholResult res = tile_chol_inv(sA, sB, sScr, mv);
+    chain_publish_diag(p, k + 1, res, sA, sB, tag);
+  }
+}
+
+__device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, double* sA, double* sB,
+                            int* ok_sh) {
+  const int nT = p.nT;
+  const int t0 = p.off[h], t1 = p.off[h + 1];
+  TileAcc acc;
+  int carry_rho = -1, carry_c = -1;  // tile currently held in acc (not yet stored)
+  for (int t = t0; t < t1; ++t) {
+    int op, rho, c, k;
+    flow_decode(p.tasks[t], op, rho, c, k);
+    int rows, cols;
+    double* g = flow_tile(p, rho, c, rows, cols);
+    if (op == FLOW_ZERO) {
+      const int cc = threadIdx.x & 63;
+      for (int r = threadIdx.x >> 6; r < rows; r += 4)
+        if (cc < cols) g[(int64_t)r * p.b + cc] = 0.0;
+      continue;
+    }
+    // the next task on the same tile keeps the accumulator in registers
+    bool keep = false;
+    if (t + 1 < t1) {
+      int op2, rho2, c2, k2;
+      flow_decode(p.tasks[t + 1], op2, rho2, c2, k2);
+      keep = (rho2 == rho && c2 == c && op2 != FLOW_ZERO);
+    }
+    const bool have = (carry_rho == rho && carry_c == c);
+    if (op == FLOW_UPD) {
+      const bool hand = flow_is_handoff(nT, rho, c);
+      if (!flow_wait(p, xflag(p, rho, k), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
+      if (rho != c && !flow_wait(p, xflag(p, c, k), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
+      lds_load_sys(sA, xtile(p, rho, k));
+      if (rho != c) lds_load_sys(sB, xtile(p, c, k));
+      if (!have) {
+        if (hand && k > 0) acc_load_sys(acc, rho == c ? hdiag(p, c) : hsub(p, c));
+        else acc_load(acc, g, p.b, rows, cols);
+      }
+      __syncthreads();
+      mma_nt<true>(acc, sA, rho == c ? sA : sB, 0, 64);
+      __syncthreads();
+      if (hand) {
+        // hand-off tiles are never carried: every update goes to the buffer
+        acc_store_sys(acc, rho == c ? hdiag(p, c) : hsub(p, c));
+        carry_rho = carry_c = -1;
+        if (rho == c + 1 && k == c - 1) flow_publish(hflag(p, c, 0), tag);
+        else if (rho == c && k == c - 2) flow_publish(hflag(p, c - 1, 1), tag);
+      } else if (keep) {
+        carry_rho = rho;
+        carry_c = c;
+      } else {
+        acc_store(acc, g, p.b, rows, cols);
+        carry_rho = carry_c = -1;
+      }
+    } else {  // FLOW_TRSM
+      if (!flow_wait(p, wflag(p, k), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 8);
+      if (!have) acc_load(acc, g, p.b, rows, cols);
+      acc_to_lds(acc, sA);
+      lds_load_sys(sB, p.winv + (int64_t)k * 4096);
+      __syncthreads();
+      TileAcc x;
+      flow_trsm(x, sA, sB);
+      __syncthreads();
+      acc_store(x, g, p.b, rows, cols);
+      acc_store_sys(x, xtile(p, rho, c));
+      carry_rho = carry_c = -1;
+      flow_publish(xflag(p, rho, c), tag);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow(FlowArgs p) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  __shared__ double sScr[4 * 64 + 64];
+  __shared__ int ok_sh;
+  const unsigned long long tag = (p.epoch[0] << 20) | (unsigned long long)p.blk;
+  if (blockIdx.x == 0) flow_chain(p, tag, sA, sB, sScr, &ok_sh);
+  else flow_helper(p, blockIdx.x - 1, tag, sA, sB, &ok_sh);
+}
+
+// -------------------------------------------------------------------- host
+
+FlowPlan::~FlowPlan() { release(); }
+
+void FlowPlan::release() {
+  if (d_tasks) hipFree(d_tasks);
+  if (d_off) hipFree(d_off);
+  if (xbuf) hipFree(xbuf);
+  if (flags) hipFree(flags);
+  d_tasks = nullptr;
+  d_off = nullptr;
+  xbuf = nullptr;
+  flags = nullptr;
+  hacc = nullptr;
+  nT = -1;
+}
+
+int FlowPlan::build(int nT_, int nTa_, int helpers_, int delta_) {
+  release();
+  nT = nT_;
+  nTa = nTa_;
+  helpers = helpers_;
+  delta = delta_;
+  const int R = 2 * nT + nTa;
+  std::vector<uint32_t> all;
+  std::vector<int> off;
+  for (int v = 0; v < 2; ++v) {  // 0: block with L, 1: last block (no L)
+    FlowShape s{nT, v == 0 ? nT : 0, nTa};
+    FlowSchedule sc;
+    flow_build_schedule(s, helpers, delta, sc);
+    task_base[v] = (int)all.size();
+    off_base[v] = (int)off.size();
+    for (int h = 0; h <= helpers; ++h) off.push_back(sc.off[h] + task_base[v]);
+    all.insert(all.end(), sc.tasks.begin(), sc.tasks.end());
+  }
+  hipError_t e;
+  if ((e = hipMalloc(&d_tasks, std::max<size_t>(1, all.size()) * sizeof(uint32_t))) != hipSuccess) return (int)e;
+  if ((e = hipMalloc(&d_off, off.size() * sizeof(int))) != hipSuccess) return (int)e;
+  if (!all.empty() && (e = hipMemcpy(d_tasks, all.data(), all.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    return (int)e;
+  if ((e = hipMemcpy(d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return (int)e;
+  const size_t xd = ((size_t)R * nT + 2 * (size_t)nT) * 4096;
+  if ((e = hipMalloc(&xbuf, xd * sizeof(double))) != hipSuccess) return (int)e;
+  hacc = xbuf + (size_t)R * nT * 4096;
+  const size_t nf = (size_t)R * nT + 3 * (size_t)nT;
+  if ((e = hipMalloc(&flags, nf * sizeof(unsigned long long))) != hipSuccess) return (int)e;
+  if ((e = hipMemset(flags, 0, nf * sizeof(unsigned long long))) != hipSuccess) return (int)e;
+  return 0;
+}
+
+int flow_helpers_default(int device) {
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_panel_flow, BTA_THREADS, 0) != hipSuccess || per <= 0)
+    return 0;
+  // room for two factorizations in flight (prior and conditional chains)
+  return std::max(1, cus * per / 2 - 1);
+}
+
+unsigned long long flow_spin_ticks(int device, double seconds) {
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0) khz = 100000;
+  return (unsigned long long)(seconds * 1e3 * khz);
+}
+
+void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, int i, int n,
+                       int b, int a, hipStream_t st) {
+  FlowArgs p;
+  const int nT = fp.nT;
+  p.D = D + (int64_t)i * b * b;
+  p.L = (i < n - 1) ? L + (int64_t)i * b * b : nullptr;
+  p.B = (a > 0) ? B + (int64_t)i * a * b : nullptr;
+  p.winv = ws.winv + (int64_t)i * nT * 4096;
+  p.xbuf = fp.xbuf;
+  p.hacc = fp.hacc;
+  p.logpart = ws.logpart + (int64_t)i * nT;
+  p.flags = fp.flags;
+  const int v = (i < n - 1) ? 0 : 1;
+  p.tasks = fp.d_tasks;
+  p.off = fp.d_off + fp.off_base[v];
+  p.epoch = ws.epoch;
+  p.info = ws.info;
+  p.err = ws.err;
+  p.row0 = (long long)i * b;
+  p.spin_ticks = fp.spin_ticks;
+  p.b = b;
+  p.a = a;
+  p.nT = nT;
+  p.nTL = (i < n - 1) ? nT : 0;
+  p.nTa = (a > 0) ? fp.nTa : 0;
+  p.blk = i + 1;
+  hipLaunchKernelGGL(k_panel_flow, dim3(1 + fp.helpers), dim3(BTA_THREADS), 0, st, p);
+}
+
+}  // namespace bta

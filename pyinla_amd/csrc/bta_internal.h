@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "bta_device.h"
+#include "bta_flow.h"
 #include "bta_gemm.h"
 
 namespace bta {
@@ -30,6 +31,29 @@ struct TimingLog {
   hipEvent_t begin(int cls, hipStream_t st);
   void end(int cls, hipStream_t st);
 };
+
+// Schedules and exchange buffers of the persistent panel launch (bta_flow.hip)
+// for one (nT, nTa, helpers, delta); variant 0 is a block with L, variant 1
+// the last block.
+struct FlowPlan {
+  uint32_t* d_tasks = nullptr;
+  int* d_off = nullptr;
+  double* xbuf = nullptr;   // (2 nT + nTa) * nT tiles, then 2 nT hand-off tiles
+  double* hacc = nullptr;
+  unsigned long long* flags = nullptr;
+  int task_base[2] = {0, 0}, off_base[2] = {0, 0};
+  int nT = -1, nTa = 0, helpers = 0, delta = 0;
+  unsigned long long spin_ticks = 0;
+  FlowPlan() = default;
+  FlowPlan(const FlowPlan&) = delete;
+  FlowPlan& operator=(const FlowPlan&) = delete;
+  ~FlowPlan();
+  void release();
+  int build(int nT, int nTa, int helpers, int delta);
+};
+
+int flow_helpers_default(int device);
+unsigned long long flow_spin_ticks(int device, double seconds);
 
 // Per-solver device workspace.  Allocated once (never inside a hot call) for a
 // fixed (n, b, a); every enqueue function only reads and writes it.
@@ -64,6 +88,12 @@ struct Workspace {
   // Output-tile edge per GEMM class: 0 = heuristic (pick_tile), 64 or 128.
   int tile_update = 0;   // block updates of the factorization
   int tile_grouped = 0;  // block inverses and selected inversion
+  // Panel factorization: 0 = automatic, 1 = one launch per tile column,
+  // 2 = persistent dataflow launch per block (bta_flow.hip).
+  int panel_mode = 0;
+  int flow_helpers = 0;  // 0 = automatic
+  int flow_delta = 2;
+  std::unique_ptr<FlowPlan> flow;
 };
 
 inline int choose_tile(int pref, int m, int n) { return pref ? pref : pick_tile(m, n); }
@@ -118,6 +148,8 @@ struct BatchPlan {
 // Builds the grouped block-update plan when the blocks are large enough for
 // 128x128 tiles (else the by-value update kernel is used); host only.
 int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a);
+void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, int i, int n,
+                       int b, int a, hipStream_t st);
 int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
                   hipStream_t st);
 // Host-side preparation (allocations, job tables); never called during capture.

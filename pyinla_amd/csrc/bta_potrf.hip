@@ -16,6 +16,8 @@
 //   D_{i+1} -= L_i L_i^T (lower),  B_{i+1} -= B_i L_i^T,  T -= B_i B_i^T (lower)
 // and the arrowhead tip is factored at the end.  Per-tile log-pivot sums are
 // reduced in a fixed order, so the log-determinant is deterministic.
+#include <algorithm>
+
 #include "bta_internal.h"
 #include "bta_tile_chol.h"
 
@@ -420,7 +422,35 @@ static void launch_gemm_jobs(UpdJobs& jobs, hipStream_t st) {
   if (total > 0) hipLaunchKernelGGL(k_gemm_nt_sub, dim3(total), dim3(BTA_THREADS), 0, st, jobs);
 }
 
+static bool use_flow(const Workspace& ws, int b) {
+  if (ws.panel_mode == 1) return false;
+  if (ws.panel_mode == 2) return true;
+  return b > 64;  // a single tile column has no panel dataflow to overlap
+}
+
+static int flow_prepare(Workspace& ws, int b, int a) {
+  const int nT = (b + 63) / 64, nTa = (a + 63) / 64;
+  int helpers = ws.flow_helpers > 0 ? ws.flow_helpers : flow_helpers_default(ws.device);
+  if (helpers <= 0) return (int)hipErrorInvalidValue;
+  helpers = std::min(helpers, 1023);
+  if (ws.flow && ws.flow->nT == nT && ws.flow->nTa == nTa && ws.flow->helpers == helpers &&
+      ws.flow->delta == ws.flow_delta)
+    return 0;
+  ws.flow.reset(new FlowPlan());
+  const int rc = ws.flow->build(nT, nTa, helpers, ws.flow_delta);
+  if (rc) {
+    ws.flow.reset();
+    return rc;
+  }
+  ws.flow->spin_ticks = flow_spin_ticks(ws.device, 20.0);
+  return 0;
+}
+
 int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a) {
+  if (use_flow(ws, b)) {
+    const int rc = flow_prepare(ws, b, a);
+    if (rc) return rc;
+  }
   if (choose_tile(ws.tile_update, b, b) != 128) {
     ws.update_plan.reset();
     return 0;
@@ -469,6 +499,8 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
   for (int q = 0; q < 4 && use_plan; ++q) use_plan = (ws.update_key[q] == key[q]);
   const int nT = (b + 63) / 64;
   const int nTa = (a + 63) / 64;
+  const bool flow = use_flow(ws, b) && ws.flow && ws.flow->nT == nT && ws.flow->nTa == nTa;
+  if (use_flow(ws, b) && !flow) return (int)hipErrorNotReady;  // potrf_prepare not run for this shape
   hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, st, ws.epoch, ws.info, ws.err);
   for (int i = 0; i < n; ++i) {
     PanelArgs p;
@@ -488,10 +520,15 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     p.nT = nT;
     p.nTL = p.L ? nT : 0;
     p.nTa = p.B ? nTa : 0;
-    for (int k = 0; k < nT; ++k) {
-      p.k = k;
+    if (flow) {
       TimedScope ts(ws, KC_PANEL, st);
-      hipLaunchKernelGGL(k_panel_step, dim3(panel_grid(nT, p.nTL, p.nTa, k)), dim3(BTA_THREADS), 0, st, p);
+      flow_launch_block(ws, *ws.flow, D, L, B, i, n, b, a, st);
+    } else {
+      for (int k = 0; k < nT; ++k) {
+        p.k = k;
+        TimedScope ts(ws, KC_PANEL, st);
+        hipLaunchKernelGGL(k_panel_step, dim3(panel_grid(nT, p.nTL, p.nTa, k)), dim3(BTA_THREADS), 0, st, p);
+      }
     }
     UpdJobs jobs;
     jobs.njobs = 0;

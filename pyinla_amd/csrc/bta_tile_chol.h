@@ -15,6 +15,10 @@
 #pragma once
 #include "bta_device.h"
 
+#ifndef BTA_BASE_LDS_BROADCAST
+#define BTA_BASE_LDS_BROADCAST 0
+#endif
+
 struct TileCholResult {
   double logsum;  // sum of log(pivot) over the valid part (thread 0 only)
   int fail;       // first non-positive / NaN pivot in the valid part, or -1 (thread 0 only)
@@ -62,7 +66,8 @@ __device__ __forceinline__ void store16(double* __restrict__ C, int r0, int c0, 
 //    triangle (upper triangle ignored).  Rows/cols >= mv are treated as identity.
 // On exit: s holds L (lower, zero strict upper, identity outside mv),
 //          w holds L^-1 (lower), both with stride BTA_LDS_LD.
-// scratch: LDS, >= 64 + 64 doubles (pivots and their logs).
+// scratch: LDS, >= 64 + 64 doubles (pivots and their logs; the LDS-broadcast
+// base case also uses 32 doubles from offset 128 -- callers provide 320).
 // Code size matters as much as instruction count here: the workgroup that runs
 // this is on the critical path of every panel launch and starts with a cold
 // instruction cache, so only the 16-step elimination is unrolled.
@@ -95,6 +100,35 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
         e[c] = (c == i) ? 1.0 : 0.0;
       }
       double myd = 1.0;
+#if BTA_BASE_LDS_BROADCAST
+      // Column j of A and row j of E are broadcast through LDS (one store per
+      // lane, conflict-free broadcast loads); wave-synchronous, no barrier.
+      double* bc = scratch + 128;  // [16] column j of A, [16] row j of E
+#pragma unroll 1
+      for (int j = 0; j < 16; ++j) {
+        const double ajj = a[j];
+        if (lane < 16) bc[lane] = ajj;                     // A[lane][j]
+        if (lane == j) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) bc[16 + c] = e[c];  // E[j][:]
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        const double d = bc[j];
+        if (lane == 0) {
+          piv[c0 + j] = d;
+          if (c0 + j < mv && fail < 0 && !(d > 0.0)) fail = c0 + j;
+        }
+        if (i == j) myd = d;
+        const double mi = (i > j) ? ajj / d : 0.0;
+#pragma unroll 1
+        for (int l = j + 1; l < 16; ++l) a[l] = fma(-mi, bc[l], a[l]);
+#pragma unroll 1
+        for (int c = 0; c <= j; ++c) e[c] = fma(-mi, bc[16 + c], e[c]);
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+      }
+#else
       // Runtime loops with wave-uniform indices: the register arrays are
       // addressed indirectly (no scratch) and the code stays a few hundred
       // instructions long.
@@ -114,6 +148,7 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
 #pragma unroll 1
         for (int c = 0; c <= j; ++c) e[c] = fma(-mi, readlane_f64(e[c], j), e[c]);
       }
+#endif
       const double rsi = 1.0 / sqrt(myd);
       if (lane < 16) {
 #pragma unroll 4

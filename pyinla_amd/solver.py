@@ -11,6 +11,7 @@ tensors on the GPU; the arithmetic runs in libpyinla_bta.so.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
 
 import numpy as np
@@ -78,6 +79,10 @@ class SerinvSolver:
         self.t_cholesky = 0.0
         self.t_solve = 0.0
         self._factored = None  # sparsity of the factor currently held, or None
+        # PYINLA_AMD_PANEL=auto|columns|flow selects the panel variant for every solver
+        panel = os.environ.get("PYINLA_AMD_PANEL")
+        if panel:
+            self.set_panel(panel)
 
     def __del__(self):
         ws = getattr(self, "_ws", None)
@@ -105,6 +110,19 @@ class SerinvSolver:
             _lib.check(self.lib.bta_potrf(self._ws, _ptr(D), _ptr(L), _ptr(B), _ptr(T), n, b, a,
                                           _stream_ptr(self.device)), "bta_potrf")
         self._factored = sparsity
+
+    def set_panel(self, mode: str = "auto", helpers: int = 0, delta: int | None = None) -> None:
+        """Panel factorization variant of ``factorize`` (no reference
+        counterpart): "auto", "columns" (one launch per 64-wide tile column)
+        or "flow" (one persistent dataflow launch per block); ``helpers`` and
+        ``delta`` tune the flow launch (0 = automatic helper count)."""
+        modes = {"auto": 0, "columns": 1, "flow": 2}
+        if mode not in modes:
+            raise ValueError(f"unknown panel mode {mode!r}")
+        _lib.check(self.lib.bta_ws_set_option(self._ws, 3, modes[mode]), "set_option")
+        _lib.check(self.lib.bta_ws_set_option(self._ws, 4, int(helpers)), "set_option")
+        if delta is not None:
+            _lib.check(self.lib.bta_ws_set_option(self._ws, 5, int(delta)), "set_option")
 
     def status(self):
         """(info, logdet, err) of the last factorization (synchronises)."""
