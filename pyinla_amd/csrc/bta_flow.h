@@ -21,6 +21,14 @@
 // deadlock.  δ > 0 defers updates of tiles far from the diagonal, which
 // evens out the per-column load of the helpers.
 //
+// The block update that couples block i to the next one is folded into the
+// same launch: as soon as the columns k..k+S-1 of L_i (and B_i) are final,
+// helpers apply D_{i+1} -= L_i L_i^T, B_{i+1} -= B_i L_i^T and T -= B_i B_i^T
+// for those S columns to the tiles they own (op FLOW_NEXT, S = FLOW_SPAN).
+// Nothing in the launch waits on these tasks, so they take the lowest
+// priority of their level and fill the time the helpers would otherwise
+// spend waiting for the chain; the next block's launch reads the results.
+//
 // Memory protocol (the L2 caches of the XCDs are not coherent): every address
 // is stored to by one workgroup only.  Helpers accumulate their tiles in place
 // with ordinary loads and stores; a finished tile is also written once, with
@@ -36,7 +44,10 @@
 
 namespace bta {
 
-enum : int { FLOW_UPD = 0, FLOW_TRSM = 1, FLOW_ZERO = 2 };
+enum : int { FLOW_UPD = 0, FLOW_TRSM = 1, FLOW_ZERO = 2, FLOW_NEXT = 3 };
+
+// Columns of L_i per next-block update task.
+#define FLOW_SPAN 4
 
 // Task word: op (2 bits) | ρ (10) | c (10) | k (10).
 __host__ __device__ inline uint32_t flow_encode(int op, int rho, int c, int k) {
@@ -50,7 +61,10 @@ __host__ __device__ inline void flow_decode(uint32_t w, int& op, int& rho, int& 
 }
 
 // Shape of a panel: nT tile columns (and D tile rows), nTL L tile rows (0 for
-// the last block), nTa arrow tile rows (0 or 1).
+// the last block), nTa arrow tile rows (0 or 1).  With nTL > 0 the launch also
+// updates D_{i+1} (and B_{i+1}); with nTa > 0 it updates the tip T.
+// FLOW_NEXT targets: row rho < nT is D_{i+1} tile (rho, c); rho == nT is the
+// arrow tile (0, c) of B_{i+1}; rho == nT + 1 is the tip T (c = 0).
 struct FlowShape {
   int nT, nTL, nTa;
   int rows() const { return nT + nTL + nTa; }

@@ -45,6 +45,24 @@ void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedul
                                    (unsigned long long)k;
     v.push_back({key, flow_encode(op, rho, c, k), flow_owner(s, rho, c, helpers)});
   };
+  // next-block updates: one task per target tile and span of S columns of
+  // L_i / B_i, at the level of the span's last column, lowest priority
+  auto push_next = [&](int rho, int c, int owner_rho, int owner_c) {
+    for (int k = 0; k < nT; k += FLOW_SPAN) {
+      const int klast = std::min(k + FLOW_SPAN, nT) - 1;
+      const unsigned long long key = ((unsigned long long)(3 * klast + 2) << 42) | (1023ull << 31) |
+                                     ((unsigned long long)c << 21) | ((unsigned long long)rho << 10) |
+                                     (unsigned long long)k;
+      v.push_back({key, flow_encode(FLOW_NEXT, rho, c, k), flow_owner(s, owner_rho, owner_c, helpers)});
+    }
+  };
+  if (s.nTL > 0) {
+    for (int r = 0; r < nT; ++r)
+      for (int c = 0; c <= r; ++c) push_next(r, c, r, c);
+    if (s.nTa > 0)
+      for (int c = 0; c < nT; ++c) push_next(nT, c, nT + s.nTL, c);
+  }
+  if (s.nTa > 0) push_next(nT + 1, 0, R - 1, nT - 1);
   for (int rho = 0; rho < R; ++rho) {
     for (int c = 0; c < nT; ++c) {
       if (rho < nT && rho < c) {  // strict upper tile of D: cleared
@@ -82,6 +100,9 @@ struct FlowArgs {
   double* D;                 // D_i (b x b)
   double* L;                 // L_i or nullptr
   double* B;                 // B_i (a x b) or nullptr
+  double* Dn;                // D_{i+1} or nullptr (last block)
+  double* Bn;                // B_{i+1} or nullptr
+  double* T;                 // tip (a x a) or nullptr
   double* winv;              // nT tiles: inverse of each diagonal tile of D_i
   double* xbuf;              // R * nT tiles: exchange copy of every finished tile
   double* hacc;              // 2 * nT tiles: hand-off accumulators (sub-diagonal, diagonal)
@@ -284,12 +305,53 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
         if (cc < cols) g[(int64_t)r * p.b + cc] = 0.0;
       continue;
     }
+    if (op == FLOW_NEXT) {
+      // target and its operand rows in this panel (stacked rows of L_i, B_i)
+      double* tg;
+      int64_t ld;
+      int trows, tcols, ra, rb;
+      if (rho < nT) {  // D_{i+1}(rho, c) -= L_i(rho, :) L_i(c, :)^T
+        tg = p.Dn + (int64_t)(64 * rho) * p.b + 64 * c;
+        ld = p.b;
+        trows = min(64, p.b - 64 * rho);
+        tcols = min(64, p.b - 64 * c);
+        ra = nT + rho;
+        rb = nT + c;
+      } else if (rho == nT) {  // B_{i+1}(0, c) -= B_i L_i(c, :)^T
+        tg = p.Bn + 64 * c;
+        ld = p.b;
+        trows = p.a;
+        tcols = min(64, p.b - 64 * c);
+        ra = nT + p.nTL;
+        rb = nT + c;
+      } else {  // T -= B_i B_i^T
+        tg = p.T;
+        ld = p.a;
+        trows = p.a;
+        tcols = p.a;
+        ra = rb = nT + p.nTL;
+      }
+      acc_load(acc, tg, ld, trows, tcols);
+      const int kend = min(k + FLOW_SPAN, nT);
+      for (int kk = k; kk < kend; ++kk) {
+        if (!flow_wait(p, xflag(p, ra, kk), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 16);
+        if (rb != ra && !flow_wait(p, xflag(p, rb, kk), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 16);
+        lds_load_sys(sA, xtile(p, ra, kk));
+        if (rb != ra) lds_load_sys(sB, xtile(p, rb, kk));
+        __syncthreads();
+        mma_nt<true>(acc, sA, rb == ra ? sA : sB, 0, 64);
+        __syncthreads();
+      }
+      acc_store(acc, tg, ld, trows, tcols);
+      carry_rho = carry_c = -1;
+      continue;
+    }
     // the next task on the same tile keeps the accumulator in registers
     bool keep = false;
     if (t + 1 < t1) {
       int op2, rho2, c2, k2;
       flow_decode(p.tasks[t + 1], op2, rho2, c2, k2);
-      keep = (rho2 == rho && c2 == c && op2 != FLOW_ZERO);
+      keep = (rho2 == rho && c2 == c && (op2 == FLOW_UPD || op2 == FLOW_TRSM));
     }
     const bool have = (carry_rho == rho && carry_c == c);
     if (op == FLOW_UPD) {
@@ -412,13 +474,16 @@ unsigned long long flow_spin_ticks(int device, double seconds) {
   return (unsigned long long)(seconds * 1e3 * khz);
 }
 
-void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, int i, int n,
-                       int b, int a, hipStream_t st) {
+void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, double* T, int i,
+                       int n, int b, int a, hipStream_t st) {
   FlowArgs p;
   const int nT = fp.nT;
   p.D = D + (int64_t)i * b * b;
   p.L = (i < n - 1) ? L + (int64_t)i * b * b : nullptr;
   p.B = (a > 0) ? B + (int64_t)i * a * b : nullptr;
+  p.Dn = (i < n - 1) ? D + (int64_t)(i + 1) * b * b : nullptr;
+  p.Bn = (a > 0 && i < n - 1) ? B + (int64_t)(i + 1) * a * b : nullptr;
+  p.T = (a > 0) ? T : nullptr;
   p.winv = ws.winv + (int64_t)i * nT * 4096;
   p.xbuf = fp.xbuf;
   p.hacc = fp.hacc;

@@ -157,20 +157,25 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __r
 // registers while the current one is multiplied.
 
 #define BIG_TS 128
+#ifndef BIG_KC
 #define BIG_KC 16
-#define BIG_LD (BIG_KC + 2)  // 18 doubles: 16 consecutive rows hit distinct LDS banks
+#endif
+// odd row stride in doubles: the 16 rows of an MFMA operand fetch start on
+// 16 distinct even dword banks, so the 64-bit reads are conflict-free
+#define BIG_LD (BIG_KC + 1)
 
 struct BigOp {
-  double r[BIG_TS * BIG_KC / BTA_THREADS];  // 8
+  static constexpr int N = BIG_TS * BIG_KC / BTA_THREADS;
+  double r[N];
   // trans == 0: X row-major rows x K -> element (i, k) = X[i][k]
   // trans == 1: X row-major K x rows -> element (i, k) = X[k][i]
   __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw,
                                        int mask, int i0, int k0) {
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
+    for (int p = 0; p < N; ++p) {
       const int e = threadIdx.x + BTA_THREADS * p;
       if (trans == 0) {
-        const int i = e >> 4, k = e & 15;
+        const int i = e / BIG_KC, k = e % BIG_KC;
         r[p] = (i < rows && k < kw && !masked_out(mask, i0 + i, k0 + k)) ? g[(int64_t)i * ld + k] : 0.0;
       } else {
         const int k = e >> 7, i = e & 127;
@@ -180,9 +185,9 @@ struct BigOp {
   }
   __device__ __forceinline__ void store(double* __restrict__ s, int trans) const {
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
+    for (int p = 0; p < N; ++p) {
       const int e = threadIdx.x + BTA_THREADS * p;
-      if (trans == 0) s[(e >> 4) * BIG_LD + (e & 15)] = r[p];
+      if (trans == 0) s[(e / BIG_KC) * BIG_LD + (e % BIG_KC)] = r[p];
       else s[(e & 127) * BIG_LD + (e >> 7)] = r[p];
     }
   }

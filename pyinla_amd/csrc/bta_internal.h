@@ -148,8 +148,10 @@ struct BatchPlan {
 // Builds the grouped block-update plan when the blocks are large enough for
 // 128x128 tiles (else the by-value update kernel is used); host only.
 int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a);
-void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, int i, int n,
-                       int b, int a, hipStream_t st);
+// One persistent launch: panel of block i plus its updates of D_{i+1},
+// B_{i+1} and T.
+void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, double* T, int i,
+                       int n, int b, int a, hipStream_t st);
 int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
                   hipStream_t st);
 // Host-side preparation (allocations, job tables); never called during capture.

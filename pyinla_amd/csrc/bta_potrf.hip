@@ -520,9 +520,10 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     p.nT = nT;
     p.nTL = p.L ? nT : 0;
     p.nTa = p.B ? nTa : 0;
-    if (flow) {
+    if (flow) {  // the launch also applies the block update
       TimedScope ts(ws, KC_PANEL, st);
-      flow_launch_block(ws, *ws.flow, D, L, B, i, n, b, a, st);
+      flow_launch_block(ws, *ws.flow, D, L, B, T, i, n, b, a, st);
+      continue;
     } else {
       for (int k = 0; k < nT; ++k) {
         p.k = k;

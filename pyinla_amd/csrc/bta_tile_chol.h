@@ -20,6 +20,23 @@
 #pragma once
 #include "bta_device.h"
 
+// 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps (full
+// FP64 accuracy for normal positive x; a non-positive pivot still propagates
+// as a negative / NaN value and is reported by the caller).
+__device__ __forceinline__ double rcp_f64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ __forceinline__ double rsq_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
+}
+
 struct TileCholResult {
   double logsum;  // sum of log(pivot) over the valid part (thread 0 only)
   int fail;       // first non-positive / NaN pivot in the valid part, or -1 (thread 0 only)
@@ -93,66 +110,69 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
 #pragma unroll 1
     for (int s4 = 0; s4 < 4; ++s4) {
       const int p = 16 * ts + 4 * s4;
-      // ---- phase A: 4x4 diagonal factor (every thread) and its inverse
+      // ---- phase A.  Each thread first issues the LDS reads of its own
+      // operands (a row of the panel below, or a column of the panel rows of
+      // E), so that their latency overlaps the 4x4 factorization.
+      const int nrow = 60 - p;  // rows below the panel
+      const int q = tid & 3;
+      const bool solve_row = tid < 4 * nrow;
+      double* opnd = solve_row ? s + (p + 4 + (tid >> 2)) * BTA_LDS_LD + p
+                               : w + p * BTA_LDS_LD + ((tid - 4 * nrow) >> 2);
+      const int ostep = solve_row ? 1 : BTA_LDS_LD;
+      const double x0 = opnd[0], x1 = opnd[ostep], x2 = opnd[2 * ostep], x3 = opnd[3 * ostep];
       const double* dg = s + p * BTA_LDS_LD + p;
       const double a00 = dg[0];
       const double a10 = dg[BTA_LDS_LD], a11 = dg[BTA_LDS_LD + 1];
       const double a20 = dg[2 * BTA_LDS_LD], a21 = dg[2 * BTA_LDS_LD + 1], a22 = dg[2 * BTA_LDS_LD + 2];
       const double a30 = dg[3 * BTA_LDS_LD], a31 = dg[3 * BTA_LDS_LD + 1], a32 = dg[3 * BTA_LDS_LD + 2];
       const double a33 = dg[3 * BTA_LDS_LD + 3];
-      const double l00 = sqrt(a00), r0 = 1.0 / l00;
-      const double l10 = a10 * r0, l20 = a20 * r0, l30 = a30 * r0;
-      const double d1 = a11 - l10 * l10;
-      const double l11 = sqrt(d1), r1 = 1.0 / l11;
-      const double l21 = (a21 - l20 * l10) * r1, l31 = (a31 - l30 * l10) * r1;
-      const double d2 = a22 - l20 * l20 - l21 * l21;
-      const double l22 = sqrt(d2), r2 = 1.0 / l22;
-      const double l32 = (a32 - l30 * l20 - l31 * l21) * r2;
-      const double d3 = a33 - l30 * l30 - l31 * l31 - l32 * l32;
-      const double l33 = sqrt(d3), r3 = 1.0 / l33;
-      // inv(l), lower: i_qq = r_q, i_qm = -r_q sum_{m<=k<q} l_qk i_km
-      const double i10 = -r1 * (l10 * r0);
-      const double i21 = -r2 * (l21 * r1);
-      const double i20 = -r2 * (l20 * r0 + l21 * i10);
-      const double i32 = -r3 * (l32 * r2);
-      const double i31 = -r3 * (l31 * r1 + l32 * i21);
-      const double i30 = -r3 * (l30 * r0 + l31 * i10 + l32 * i20);
+      // LDL^T of the 4x4 block: the dependent chain holds one reciprocal per
+      // column; square roots come after, off the chain.  l = u diag(sqrt d).
+      const double d0 = a00, q0 = rcp_f64(d0);
+      const double u10 = a10 * q0, u20 = a20 * q0, u30 = a30 * q0;
+      const double d1 = fma(-u10, a10, a11), q1 = rcp_f64(d1);
+      const double t21 = fma(-u20, a10, a21), t31 = fma(-u30, a10, a31);
+      const double u21 = t21 * q1, u31 = t31 * q1;
+      const double d2 = fma(-u21, t21, fma(-u20, a20, a22)), q2 = rcp_f64(d2);
+      const double t32 = fma(-u31, t21, fma(-u30, a20, a32));
+      const double u32 = t32 * q2;
+      const double d3 = fma(-u32, t32, fma(-u31, t31, fma(-u30, a30, a33)));
+      const double g0 = rsq_f64(d0), g1 = rsq_f64(d1), g2 = rsq_f64(d2), g3 = rsq_f64(d3);
+      const double s0 = d0 * g0, s1 = d1 * g1, s2 = d2 * g2, s3 = d3 * g3;  // sqrt(d)
+      // inv(l) = diag(1/sqrt d) inv(u), inv(u) unit lower
+      const double v10 = -u10, v21 = -u21, v32 = -u32;
+      const double v20 = -fma(u21, v10, u20);
+      const double v31 = -fma(u32, v21, u31);
+      const double v30 = -fma(u32, v20, fma(u31, v10, u30));
+      const double r0 = g0, r1 = g1, r2 = g2, r3 = g3;
+      const double i10 = g1 * v10, i20 = g2 * v20, i21 = g2 * v21;
+      const double i30 = g3 * v30, i31 = g3 * v31, i32 = g3 * v32;
+      const double l00 = s0, l10 = u10 * s0, l20 = u20 * s0, l30 = u30 * s0;
+      const double l11 = s1, l21 = u21 * s1, l31 = u31 * s1;
+      const double l22 = s2, l32 = u32 * s2, l33 = s3;
       if (tid == 0) {
-        piv[p] = a00;
+        piv[p] = d0;
         piv[p + 1] = d1;
         piv[p + 2] = d2;
         piv[p + 3] = d3;
         if (fail < 0) {
-          if (p < mv && !(a00 > 0.0)) fail = p;
+          if (p < mv && !(d0 > 0.0)) fail = p;
           else if (p + 1 < mv && !(d1 > 0.0)) fail = p + 1;
           else if (p + 2 < mv && !(d2 > 0.0)) fail = p + 2;
           else if (p + 3 < mv && !(d3 > 0.0)) fail = p + 3;
         }
       }
-      const int nrow = 60 - p;  // rows below the panel
-      const int q = tid & 3;
-      if (tid < 4 * nrow) {
-        // X[r][q] = sum_{m<=q} A[r][p+m] inv(l)[q][m]; the 4 threads of a row
-        // are adjacent lanes of one wave, so all reads precede the write
-        double* row = s + (p + 4 + (tid >> 2)) * BTA_LDS_LD + p;
-        const double x0 = row[0], x1 = row[1], x2 = row[2], x3 = row[3];
+      // rows below: X[r][q] = sum_{m<=q} A[r][p+m] inv(l)[q][m];
+      // panel rows of E: E[p+q][c] = sum_{m<=q} inv(l)[q][m] E[p+m][c] (c <= p+3).
+      // The 4 threads sharing a row / column are adjacent lanes of one wave,
+      // so all of their reads precede the write.
+      {
         double v;
         if (q == 0) v = x0 * r0;
-        else if (q == 1) v = x0 * i10 + x1 * r1;
-        else if (q == 2) v = x0 * i20 + x1 * i21 + x2 * r2;
-        else v = x0 * i30 + x1 * i31 + x2 * i32 + x3 * r3;
-        row[q] = v;
-      } else {
-        // E[p+q][c] = sum_{m<=q} inv(l)[q][m] E[p+m][c], c <= p + 3
-        const int c = (tid - 4 * nrow) >> 2;
-        double* col = w + p * BTA_LDS_LD + c;
-        const double e0 = col[0], e1 = col[BTA_LDS_LD], e2 = col[2 * BTA_LDS_LD], e3 = col[3 * BTA_LDS_LD];
-        double v;
-        if (q == 0) v = r0 * e0;
-        else if (q == 1) v = i10 * e0 + r1 * e1;
-        else if (q == 2) v = i20 * e0 + i21 * e1 + r2 * e2;
-        else v = i30 * e0 + i31 * e1 + i32 * e2 + r3 * e3;
-        col[q * BTA_LDS_LD] = v;
+        else if (q == 1) v = fma(x0, i10, x1 * r1);
+        else if (q == 2) v = fma(x0, i20, fma(x1, i21, x2 * r2));
+        else v = fma(x0, i30, fma(x1, i31, fma(x2, i32, x3 * r3)));
+        opnd[q * ostep] = v;
       }
       __syncthreads();
       // ---- phase B: write l; rank-4 updates of the tile column and of E

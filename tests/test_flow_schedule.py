@@ -11,7 +11,9 @@ published.  It checks that
   by the chain (update counts are tracked per tile),
 * every factor address is stored to by one agent only (the cache-coherence
   rule the kernel relies on),
-* the result equals numpy's Cholesky factor of the panel [D; L; B].
+* the result equals numpy's Cholesky factor of the panel [D; L; B], and the
+  next-block updates D_{i+1} -= L L^T, B_{i+1} -= B L^T, T -= B B^T folded
+  into the same launch are applied exactly once per column.
 """
 import ctypes as C
 import random
@@ -21,7 +23,8 @@ import pytest
 
 from pyinla_amd import _lib
 
-UPD, TRSM, ZERO = 0, 1, 2
+UPD, TRSM, ZERO, NEXT = 0, 1, 2, 3
+SPAN = 4
 
 
 def schedule(nT, nTL, nTa, helpers, delta):
@@ -45,7 +48,7 @@ class Sim:
     """Tile-level model of one k_panel_flow launch (64-wide tiles; rows and
     columns beyond the matrix are zero-padded as on the device)."""
 
-    def __init__(self, D, L, B, ts=64):
+    def __init__(self, D, L, B, ts=64, Dn=None, Bn=None, T=None):
         self.ts = ts
         b = D.shape[0]
         self.b = b
@@ -60,6 +63,10 @@ class Sim:
             self.P[self.nT * ts:self.nT * ts + b, :b] = L
         if self.nTa:
             self.P[(self.nT + self.nTL) * ts:(self.nT + self.nTL) * ts + B.shape[0], :b] = B
+        self.Dn = np.zeros((pad, pad)) if Dn is None else np.pad(Dn, ((0, pad - b), (0, pad - b)))
+        self.Bn = np.zeros((ts, pad)) if Bn is None else np.pad(Bn, ((0, ts - Bn.shape[0]), (0, pad - b)))
+        self.T = np.zeros((ts, ts)) if T is None else np.pad(T, ((0, ts - T.shape[0]), (0, ts - T.shape[0])))
+        self.next_cols = {}  # next-block target -> columns applied
         self.xbuf = {}
         self.hacc = {}
         self.winv = {}
@@ -126,6 +133,21 @@ class Sim:
         nT = self.nT
         name = f"h{h}"
         for op, r, c, k in tasks:
+            if op == NEXT:
+                t = self.ts
+                if r < nT:
+                    tgt, ra, rb = self.Dn[r * t:(r + 1) * t, c * t:(c + 1) * t], nT + r, nT + c
+                elif r == nT:
+                    tgt, ra, rb = self.Bn[:, c * t:(c + 1) * t], nT + self.nTL, nT + c
+                else:
+                    tgt, ra, rb = self.T, nT + self.nTL, nT + self.nTL
+                for kk in range(k, min(k + SPAN, nT)):
+                    yield ("x", ra, kk)
+                    if rb != ra:
+                        yield ("x", rb, kk)
+                    tgt -= self.xbuf[(ra, kk)] @ self.xbuf[(rb, kk)].T
+                    self.next_cols.setdefault((r, c), []).append(kk)
+                continue
             if op == ZERO:
                 self.store(name, r, c, np.zeros((self.ts, self.ts)))
                 continue
@@ -183,6 +205,30 @@ def _panel(b, a, with_l, seed):
     L = rng.normal(size=(b, b)) if with_l else None
     B = rng.normal(size=(a, b)) if a else None
     return D, L, B
+
+
+@pytest.mark.parametrize("b,a,helpers,delta", [(300, 6, 5, 2), (256, 0, 3, 0), (200, 9, 7, 1), (130, 4, 2, 3)])
+def test_schedule_simulation_next_block_updates(b, a, helpers, delta):
+    D, L, B = _panel(b, a, True, 8)
+    rng = np.random.default_rng(9)
+    Dn = rng.normal(size=(b, b))
+    Bn = rng.normal(size=(a, b)) if a else None
+    T = rng.normal(size=(a, a)) if a else None
+    nT = -(-b // 64)
+    lists = schedule(nT, nT, 1 if a else 0, helpers, delta)
+    s = Sim(D, L, B, Dn=Dn, Bn=Bn, T=T)
+    s.run(lists, 1)
+    for key, cols in s.next_cols.items():
+        assert sorted(cols) == list(range(nT)), key
+    Lf = np.linalg.cholesky(D)
+    Li = np.linalg.solve(Lf, L.T).T
+    got = s.Dn[:b, :b]
+    exp = Dn - Li @ Li.T
+    assert np.abs(np.tril(got) - np.tril(exp)).max() <= 1e-9 * np.abs(exp).max()
+    if a:
+        Bi = np.linalg.solve(Lf, B.T).T
+        np.testing.assert_allclose(s.Bn[:a, :b], Bn - Bi @ Li.T, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(s.T[:a, :a], T - Bi @ Bi.T, rtol=1e-9, atol=1e-9)
 
 
 @pytest.mark.parametrize("b,a,with_l,helpers,delta", [
