@@ -142,43 +142,55 @@ __device__ __forceinline__ void mma_nt(TileAcc& t, const double* __restrict__ sA
 }
 
 // Load / store the accumulator from / to a row-major global block (masked).
+// Addresses are formed as one per-lane base pointer plus wave-uniform offsets
+// (16x + 4r) * ld + 16y, so that only the base occupies vector registers.
 __device__ __forceinline__ void acc_load(TileAcc& t, const double* __restrict__ g, int64_t ld,
                                          int rows, int cols) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rb = (w >> 1) * 32 + (lane >> 4), cb = (w & 1) * 32 + (lane & 15);
+  const double* base = g + (int64_t)rb * ld + cb;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = acc_row(x, r), j = acc_col(y);
-        t.c[x][y][r] = (i < rows && j < cols) ? g[(int64_t)i * ld + j] : 0.0;
+        const int di = 16 * x + 4 * r;
+        t.c[x][y][r] = (rb + di < rows && cb + 16 * y < cols) ? base[(int64_t)di * ld + 16 * y] : 0.0;
       }
 }
 
 __device__ __forceinline__ void acc_store(const TileAcc& t, double* __restrict__ g, int64_t ld,
                                           int rows, int cols) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rb = (w >> 1) * 32 + (lane >> 4), cb = (w & 1) * 32 + (lane & 15);
+  double* base = g + (int64_t)rb * ld + cb;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = acc_row(x, r), j = acc_col(y);
-        if (i < rows && j < cols) g[(int64_t)i * ld + j] = t.c[x][y][r];
+        const int di = 16 * x + 4 * r;
+        if (rb + di < rows && cb + 16 * y < cols) base[(int64_t)di * ld + 16 * y] = t.c[x][y][r];
       }
 }
 
 // Store only the lower triangle (i >= j) of a diagonal tile.
 __device__ __forceinline__ void acc_store_lower(const TileAcc& t, double* __restrict__ g, int64_t ld,
                                                 int rows, int cols) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rb = (w >> 1) * 32 + (lane >> 4), cb = (w & 1) * 32 + (lane & 15);
+  double* base = g + (int64_t)rb * ld + cb;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int i = acc_row(x, r), j = acc_col(y);
-        if (i < rows && j < cols && i >= j) g[(int64_t)i * ld + j] = t.c[x][y][r];
+        const int di = 16 * x + 4 * r;
+        const int i = rb + di, j = cb + 16 * y;
+        if (i < rows && j < cols && i >= j) base[(int64_t)di * ld + 16 * y] = t.c[x][y][r];
       }
 }
 

@@ -18,17 +18,42 @@ namespace bta {
 
 // ---------------------------------------------------------------- schedule
 
-// Tiles are dealt round-robin in column-major order (column c: D rows c..nT-1,
-// then the L and arrow rows), so every helper owns tiles spread evenly over
-// the columns and its share of each column's updates stays balanced as the
-// factorization advances.  Cleared upper tiles of D go by row-major index.
+// Ownership.  The tiles on the chain's path -- (c, c), (c+1, c) and (c+2, c)
+// of D, whose updates and solves the chain waits for one or two columns later
+// -- are dealt round-robin over a small pool of "near" helpers that own
+// nothing else, so none of their tasks ever queues behind bulk work.  All
+// other tiles are dealt round-robin in column-major order (column c: D rows
+// c+3..nT-1, then the L and arrow rows) over the remaining helpers, so each of
+// them owns tiles spread evenly over the columns and its share of every
+// column's updates stays balanced; the next-block updates go to the same
+// pool.  Cleared upper tiles of D go by row-major index.
+static int flow_near_pool(const FlowShape& s, int helpers) {
+  if (helpers < 4) return 0;
+  return std::min(3 * s.nT, helpers / 4);
+}
+
+static bool flow_is_near(int nT, int rho, int c) { return rho < nT && rho >= c && rho - c <= 2; }
+
 int flow_owner(const FlowShape& s, int rho, int c, int helpers) {
   const int nT = s.nT, below = s.nTL + s.nTa;
   if (rho < nT && rho < c) return (int)(((long long)rho * nT + c) % helpers);
-  // tiles in columns 0..c-1: sum (nT - c' + below)
-  long long t = (long long)c * (nT + below) - (long long)c * (c - 1) / 2;
-  t += (rho < nT) ? rho - c : (nT - c) + (rho - nT);
-  return (int)(t % helpers);
+  const int hn = flow_near_pool(s, helpers);
+  if (hn > 0 && flow_is_near(nT, rho, c)) return (3 * c + (rho - c)) % hn;
+  long long t = 0;  // column-major index among the other tiles
+  for (int cc = 0; cc < c; ++cc) {
+    const int nd = nT - cc;
+    t += (hn > 0 ? nd - std::min(3, nd) : nd) + below;
+  }
+  if (rho < nT) t += hn > 0 ? rho - c - 3 : rho - c;
+  else t += (hn > 0 ? std::max(0, nT - c - 3) : nT - c) + (rho - nT);
+  return hn + (int)(t % (helpers - hn));
+}
+
+// Owner of a next-block update target (never a near helper).
+static int flow_next_owner(const FlowShape& s, int rho, int c, int helpers) {
+  const int hn = flow_near_pool(s, helpers);
+  const long long t = (long long)rho * (s.nT + 1) + c;
+  return hn + (int)(t % (helpers - hn));
 }
 
 void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedule& out) {
@@ -47,22 +72,22 @@ void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedul
   };
   // next-block updates: one task per target tile and span of S columns of
   // L_i / B_i, at the level of the span's last column, lowest priority
-  auto push_next = [&](int rho, int c, int owner_rho, int owner_c) {
+  auto push_next = [&](int rho, int c) {
     for (int k = 0; k < nT; k += FLOW_SPAN) {
       const int klast = std::min(k + FLOW_SPAN, nT) - 1;
       const unsigned long long key = ((unsigned long long)(3 * klast + 2) << 42) | (1023ull << 31) |
                                      ((unsigned long long)c << 21) | ((unsigned long long)rho << 10) |
                                      (unsigned long long)k;
-      v.push_back({key, flow_encode(FLOW_NEXT, rho, c, k), flow_owner(s, owner_rho, owner_c, helpers)});
+      v.push_back({key, flow_encode(FLOW_NEXT, rho, c, k), flow_next_owner(s, rho, c, helpers)});
     }
   };
   if (s.nTL > 0) {
     for (int r = 0; r < nT; ++r)
-      for (int c = 0; c <= r; ++c) push_next(r, c, r, c);
+      for (int c = 0; c <= r; ++c) push_next(r, c);
     if (s.nTa > 0)
-      for (int c = 0; c < nT; ++c) push_next(nT, c, nT + s.nTL, c);
+      for (int c = 0; c < nT; ++c) push_next(nT, c);
   }
-  if (s.nTa > 0) push_next(nT + 1, 0, R - 1, nT - 1);
+  if (s.nTa > 0) push_next(nT + 1, 0);
   for (int rho = 0; rho < R; ++rho) {
     for (int c = 0; c < nT; ++c) {
       if (rho < nT && rho < c) {  // strict upper tile of D: cleared
@@ -288,112 +313,165 @@ __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA
   }
 }
 
+// One step of a helper: a task, or one column kk of a next-block task.
+struct HStep {
+  int t, kk;            // task index, column (FLOW_NEXT: current column of the span)
+  int op, rho, c, k;    // decoded task
+  const double* srcA;   // operand tiles (64 x 64, contiguous); TRSM: A is the accumulator
+  const double* srcB;
+  unsigned long long* f1;
+  unsigned long long* f2;
+};
+
+__device__ __forceinline__ void next_rows(const FlowArgs& p, int rho, int c, int& ra, int& rb) {
+  const int nT = p.nT;
+  if (rho < nT) { ra = nT + rho; rb = nT + c; }
+  else if (rho == nT) { ra = nT + p.nTL; rb = nT + c; }
+  else { ra = rb = nT + p.nTL; }
+}
+
+// Decodes task t at column kk (kk < 0: the task's first column).
+__device__ __forceinline__ HStep make_step(const FlowArgs& p, int t, int kk) {
+  HStep st;
+  st.t = t;
+  flow_decode(p.tasks[t], st.op, st.rho, st.c, st.k);
+  st.kk = kk < 0 ? st.k : kk;
+  st.srcA = st.srcB = nullptr;
+  st.f1 = st.f2 = nullptr;
+  if (st.op == FLOW_UPD) {
+    st.srcA = xtile(p, st.rho, st.k);
+    st.srcB = xtile(p, st.c, st.k);
+    st.f1 = xflag(p, st.rho, st.k);
+    if (st.rho != st.c) st.f2 = xflag(p, st.c, st.k);
+  } else if (st.op == FLOW_TRSM) {
+    st.srcB = p.winv + (int64_t)st.k * 4096;
+    st.f1 = wflag(p, st.k);
+  } else if (st.op == FLOW_NEXT) {
+    int ra, rb;
+    next_rows(p, st.rho, st.c, ra, rb);
+    st.srcA = xtile(p, ra, st.kk);
+    st.srcB = xtile(p, rb, st.kk);
+    st.f1 = xflag(p, ra, st.kk);
+    if (rb != ra) st.f2 = xflag(p, rb, st.kk);
+  }
+  return st;
+}
+
+__device__ __forceinline__ bool flag_is(const unsigned long long* f, unsigned long long tag) {
+  return f == nullptr || __hip_atomic_load(const_cast<unsigned long long*>(f), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_SYSTEM) == tag;
+}
+
+// Issue the operand loads of a step into registers.  The exchange tiles and
+// tile inverses are written once per launch (before their flag), so ordinary
+// cached loads after the flag see them; the L2 lines cannot be older than the
+// launch.
+__device__ __forceinline__ void step_fetch(const HStep& st, TilePrefetch& pa, TilePrefetch& pb) {
+  if (st.srcA) pa.load(st.srcA, 64, 0, 64, 64);
+  if (st.srcB && st.srcB != st.srcA) pb.load(st.srcB, 64, 0, 64, 64);
+}
+
 __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, double* sA, double* sB,
                             int* ok_sh) {
   const int nT = p.nT;
   const int t0 = p.off[h], t1 = p.off[h + 1];
   TileAcc acc;
-  int carry_rho = -1, carry_c = -1;  // tile currently held in acc (not yet stored)
-  for (int t = t0; t < t1; ++t) {
-    int op, rho, c, k;
-    flow_decode(p.tasks[t], op, rho, c, k);
-    int rows, cols;
-    double* g = flow_tile(p, rho, c, rows, cols);
-    if (op == FLOW_ZERO) {
+  TilePrefetch pa, pb;
+  int carry_rho = -1, carry_c = -1;  // in-panel tile currently held in acc
+  bool fetched = false;              // operands of `cur` already in pa / pb
+  if (t0 >= t1) return;
+  HStep cur = make_step(p, t0, -1);
+  for (;;) {
+    int rows = 0, cols = 0;
+    double* g = nullptr;
+    if (cur.op != FLOW_NEXT) g = flow_tile(p, cur.rho, cur.c, rows, cols);
+    if (cur.op == FLOW_ZERO) {
       const int cc = threadIdx.x & 63;
       for (int r = threadIdx.x >> 6; r < rows; r += 4)
         if (cc < cols) g[(int64_t)r * p.b + cc] = 0.0;
+      if (cur.t + 1 >= t1) break;
+      cur = make_step(p, cur.t + 1, -1);
+      fetched = false;
       continue;
     }
-    if (op == FLOW_NEXT) {
-      // target and its operand rows in this panel (stacked rows of L_i, B_i)
-      double* tg;
-      int64_t ld;
-      int trows, tcols, ra, rb;
-      if (rho < nT) {  // D_{i+1}(rho, c) -= L_i(rho, :) L_i(c, :)^T
-        tg = p.Dn + (int64_t)(64 * rho) * p.b + 64 * c;
-        ld = p.b;
-        trows = min(64, p.b - 64 * rho);
-        tcols = min(64, p.b - 64 * c);
-        ra = nT + rho;
-        rb = nT + c;
-      } else if (rho == nT) {  // B_{i+1}(0, c) -= B_i L_i(c, :)^T
-        tg = p.Bn + 64 * c;
-        ld = p.b;
+    if (!fetched) {
+      if (!flow_wait(p, cur.f1, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
+      if (cur.f2 && !flow_wait(p, cur.f2, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
+      step_fetch(cur, pa, pb);
+    }
+    // accumulator of the step's target
+    double* tg = g;
+    int64_t ld = p.b;
+    int trows = rows, tcols = cols;
+    bool hand = false;
+    if (cur.op == FLOW_NEXT) {
+      if (cur.rho < nT) {
+        tg = p.Dn + (int64_t)(64 * cur.rho) * p.b + 64 * cur.c;
+        trows = min(64, p.b - 64 * cur.rho);
+        tcols = min(64, p.b - 64 * cur.c);
+      } else if (cur.rho == nT) {
+        tg = p.Bn + 64 * cur.c;
         trows = p.a;
-        tcols = min(64, p.b - 64 * c);
-        ra = nT + p.nTL;
-        rb = nT + c;
-      } else {  // T -= B_i B_i^T
+        tcols = min(64, p.b - 64 * cur.c);
+      } else {
         tg = p.T;
         ld = p.a;
-        trows = p.a;
-        tcols = p.a;
-        ra = rb = nT + p.nTL;
+        trows = tcols = p.a;
       }
-      acc_load(acc, tg, ld, trows, tcols);
-      const int kend = min(k + FLOW_SPAN, nT);
-      for (int kk = k; kk < kend; ++kk) {
-        if (!flow_wait(p, xflag(p, ra, kk), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 16);
-        if (rb != ra && !flow_wait(p, xflag(p, rb, kk), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 16);
-        lds_load_sys(sA, xtile(p, ra, kk));
-        if (rb != ra) lds_load_sys(sB, xtile(p, rb, kk));
-        __syncthreads();
-        mma_nt<true>(acc, sA, rb == ra ? sA : sB, 0, 64);
-        __syncthreads();
-      }
-      acc_store(acc, tg, ld, trows, tcols);
-      carry_rho = carry_c = -1;
-      continue;
-    }
-    // the next task on the same tile keeps the accumulator in registers
-    bool keep = false;
-    if (t + 1 < t1) {
-      int op2, rho2, c2, k2;
-      flow_decode(p.tasks[t + 1], op2, rho2, c2, k2);
-      keep = (rho2 == rho && c2 == c && (op2 == FLOW_UPD || op2 == FLOW_TRSM));
-    }
-    const bool have = (carry_rho == rho && carry_c == c);
-    if (op == FLOW_UPD) {
-      const bool hand = flow_is_handoff(nT, rho, c);
-      if (!flow_wait(p, xflag(p, rho, k), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
-      if (rho != c && !flow_wait(p, xflag(p, c, k), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
-      lds_load_sys(sA, xtile(p, rho, k));
-      if (rho != c) lds_load_sys(sB, xtile(p, c, k));
-      if (!have) {
-        if (hand && k > 0) acc_load_sys(acc, rho == c ? hdiag(p, c) : hsub(p, c));
+      if (cur.kk == cur.k) acc_load(acc, tg, ld, trows, tcols);
+    } else {
+      hand = (cur.op == FLOW_UPD) && flow_is_handoff(nT, cur.rho, cur.c);
+      if (!(carry_rho == cur.rho && carry_c == cur.c)) {
+        if (hand && cur.k > 0) acc_load_sys(acc, cur.rho == cur.c ? hdiag(p, cur.c) : hsub(p, cur.c));
         else acc_load(acc, g, p.b, rows, cols);
       }
+    }
+    // stage operands in LDS
+    if (cur.op == FLOW_TRSM) acc_to_lds(acc, sA);
+    else pa.store(sA, 0);
+    if (cur.srcB != cur.srcA) pb.store(sB, 0);
+    // the following step; its operands are fetched now if they are ready
+    HStep nxt;
+    bool has_next = true;
+    if (cur.op == FLOW_NEXT && cur.kk + 1 < min(cur.k + FLOW_SPAN, nT)) nxt = make_step(p, cur.t, cur.kk + 1);
+    else if (cur.t + 1 < t1) nxt = make_step(p, cur.t + 1, -1);
+    else has_next = false;
+    if (threadIdx.x == 0)
+      *ok_sh = (has_next && nxt.op != FLOW_ZERO && flag_is(nxt.f1, tag) && flag_is(nxt.f2, tag)) ? 1 : 0;
+    __syncthreads();
+    fetched = (*ok_sh != 0);
+    if (fetched) step_fetch(nxt, pa, pb);
+    const double* sBop = (cur.srcB == cur.srcA) ? sA : sB;
+    if (cur.op == FLOW_TRSM) {
+      flow_trsm(acc, sA, sB);  // the accumulator was staged in sA: reuse its registers
       __syncthreads();
-      mma_nt<true>(acc, sA, rho == c ? sA : sB, 0, 64);
+      acc_store(acc, g, p.b, rows, cols);
+      acc_store_sys(acc, xtile(p, cur.rho, cur.c));
+      carry_rho = carry_c = -1;
+      flow_publish(xflag(p, cur.rho, cur.c), tag);
+    } else {
+      mma_nt<true>(acc, sA, sBop, 0, 64);
       __syncthreads();
-      if (hand) {
-        // hand-off tiles are never carried: every update goes to the buffer
-        acc_store_sys(acc, rho == c ? hdiag(p, c) : hsub(p, c));
+      if (cur.op == FLOW_NEXT) {
+        if (!(has_next && nxt.op == FLOW_NEXT && nxt.t == cur.t)) acc_store(acc, tg, ld, trows, tcols);
         carry_rho = carry_c = -1;
-        if (rho == c + 1 && k == c - 1) flow_publish(hflag(p, c, 0), tag);
-        else if (rho == c && k == c - 2) flow_publish(hflag(p, c - 1, 1), tag);
-      } else if (keep) {
-        carry_rho = rho;
-        carry_c = c;
+      } else if (hand) {
+        // hand-off tiles are never carried: every update goes to the buffer
+        acc_store_sys(acc, cur.rho == cur.c ? hdiag(p, cur.c) : hsub(p, cur.c));
+        carry_rho = carry_c = -1;
+        if (cur.rho == cur.c + 1 && cur.k == cur.c - 1) flow_publish(hflag(p, cur.c, 0), tag);
+        else if (cur.rho == cur.c && cur.k == cur.c - 2) flow_publish(hflag(p, cur.c - 1, 1), tag);
+      } else if (has_next && nxt.rho == cur.rho && nxt.c == cur.c &&
+                 (nxt.op == FLOW_UPD || nxt.op == FLOW_TRSM)) {
+        carry_rho = cur.rho;  // the next task on this tile keeps the accumulator
+        carry_c = cur.c;
       } else {
         acc_store(acc, g, p.b, rows, cols);
         carry_rho = carry_c = -1;
       }
-    } else {  // FLOW_TRSM
-      if (!flow_wait(p, wflag(p, k), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 8);
-      if (!have) acc_load(acc, g, p.b, rows, cols);
-      acc_to_lds(acc, sA);
-      lds_load_sys(sB, p.winv + (int64_t)k * 4096);
-      __syncthreads();
-      TileAcc x;
-      flow_trsm(x, sA, sB);
-      __syncthreads();
-      acc_store(x, g, p.b, rows, cols);
-      acc_store_sys(x, xtile(p, rho, c));
-      carry_rho = carry_c = -1;
-      flow_publish(xflag(p, rho, c), tag);
     }
+    if (!has_next) break;
+    cur = nxt;
   }
 }
 
