@@ -283,7 +283,7 @@ int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B
   bta_batch* bt = new bta_batch();
   bt->device = ws[0]->w.device;
   bt->ws.assign(ws, ws + nprob);
-  const int rc = bt->plan.build(ps);
+  const int rc = bt->plan.build(ps, ws[0]->w.panel_mode, ws[0]->w.device);
   if (rc) {
     delete bt;
     return rc;

@@ -85,6 +85,31 @@ struct FlowSchedule {
   int helpers = 0;
 };
 
+// Arguments of one panel launch (one problem's block i).
+struct FlowArgs {
+  double* D;                 // D_i (b x b)
+  double* L;                 // L_i or nullptr
+  double* B;                 // B_i (a x b) or nullptr
+  double* Dn;                // D_{i+1} or nullptr (last block)
+  double* Bn;                // B_{i+1} or nullptr
+  double* T;                 // tip (a x a) or nullptr
+  double* winv;              // nT tiles: inverse of each diagonal tile of D_i
+  double* xbuf;              // R * nT tiles: exchange copy of every finished tile
+  double* hacc;              // 2 * nT tiles: hand-off accumulators (sub-diagonal, diagonal)
+  double* logpart;           // nT partial log sums
+  unsigned long long* flags; // R*nT finished tiles, nT tile inverses, 2*nT hand-offs
+  const uint32_t* tasks;
+  const int* off;            // helpers + 1
+  const unsigned long long* epoch;
+  unsigned long long* info;
+  int* err;
+  long long row0;
+  unsigned long long spin_ticks;  // bound on one wait, in s_memrealtime ticks
+  int b, a, nT, nTL, nTa;
+  int blk;                   // block index + 1 (part of the flag tag)
+  int helpers;               // helper workgroups of this launch entry
+};
+
 int flow_owner(const FlowShape& s, int rho, int c, int helpers);
 void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedule& out);
 

@@ -121,28 +121,6 @@ void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedul
 
 // ------------------------------------------------------------------ device
 
-struct FlowArgs {
-  double* D;                 // D_i (b x b)
-  double* L;                 // L_i or nullptr
-  double* B;                 // B_i (a x b) or nullptr
-  double* Dn;                // D_{i+1} or nullptr (last block)
-  double* Bn;                // B_{i+1} or nullptr
-  double* T;                 // tip (a x a) or nullptr
-  double* winv;              // nT tiles: inverse of each diagonal tile of D_i
-  double* xbuf;              // R * nT tiles: exchange copy of every finished tile
-  double* hacc;              // 2 * nT tiles: hand-off accumulators (sub-diagonal, diagonal)
-  double* logpart;           // nT partial log sums
-  unsigned long long* flags; // R*nT finished tiles, nT tile inverses, 2*nT hand-offs
-  const uint32_t* tasks;
-  const int* off;            // helpers + 1
-  const unsigned long long* epoch;
-  unsigned long long* info;
-  int* err;
-  long long row0;
-  unsigned long long spin_ticks;  // bound on one wait, in s_memrealtime ticks
-  int b, a, nT, nTL, nTa;
-  int blk;                   // block index + 1 (part of the flag tag)
-};
 
 __device__ __forceinline__ double* flow_tile(const FlowArgs& p, int rho, int c, int& rows, int& cols) {
   cols = min(64, p.b - 64 * c);
@@ -487,6 +465,8 @@ __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow(FlowArgs p) {
 
 // -------------------------------------------------------------------- host
 
+int flow_capacity(int device);
+
 FlowPlan::~FlowPlan() { release(); }
 
 void FlowPlan::release() {
@@ -537,13 +517,9 @@ int FlowPlan::build(int nT_, int nTa_, int helpers_, int delta_) {
 }
 
 int flow_helpers_default(int device) {
-  int cus = 0, per = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
-    return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_panel_flow, BTA_THREADS, 0) != hipSuccess || per <= 0)
-    return 0;
   // room for two factorizations in flight (prior and conditional chains)
-  return std::max(1, cus * per / 2 - 1);
+  const int cap = flow_capacity(device);
+  return cap > 0 ? std::max(1, cap / 2 - 1) : 0;
 }
 
 unsigned long long flow_spin_ticks(int device, double seconds) {
@@ -552,8 +528,9 @@ unsigned long long flow_spin_ticks(int device, double seconds) {
   return (unsigned long long)(seconds * 1e3 * khz);
 }
 
-void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, double* T, int i,
-                       int n, int b, int a, hipStream_t st) {
+FlowArgs flow_args(const FlowPlan& fp, double* D, double* L, double* B, double* T, double* winv,
+                   double* logpart, const unsigned long long* epoch, unsigned long long* info, int* err, int i,
+                   int n, int b, int a) {
   FlowArgs p;
   const int nT = fp.nT;
   p.D = D + (int64_t)i * b * b;
@@ -562,17 +539,17 @@ void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, doubl
   p.Dn = (i < n - 1) ? D + (int64_t)(i + 1) * b * b : nullptr;
   p.Bn = (a > 0 && i < n - 1) ? B + (int64_t)(i + 1) * a * b : nullptr;
   p.T = (a > 0) ? T : nullptr;
-  p.winv = ws.winv + (int64_t)i * nT * 4096;
+  p.winv = winv + (int64_t)i * nT * 4096;
   p.xbuf = fp.xbuf;
   p.hacc = fp.hacc;
-  p.logpart = ws.logpart + (int64_t)i * nT;
+  p.logpart = logpart + (int64_t)i * nT;
   p.flags = fp.flags;
   const int v = (i < n - 1) ? 0 : 1;
   p.tasks = fp.d_tasks;
   p.off = fp.d_off + fp.off_base[v];
-  p.epoch = ws.epoch;
-  p.info = ws.info;
-  p.err = ws.err;
+  p.epoch = epoch;
+  p.info = info;
+  p.err = err;
   p.row0 = (long long)i * b;
   p.spin_ticks = fp.spin_ticks;
   p.b = b;
@@ -581,7 +558,53 @@ void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, doubl
   p.nTL = (i < n - 1) ? nT : 0;
   p.nTa = (a > 0) ? fp.nTa : 0;
   p.blk = i + 1;
+  p.helpers = fp.helpers;
+  return p;
+}
+
+void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, double* T, int i,
+                       int n, int b, int a, hipStream_t st) {
+  const FlowArgs p = flow_args(fp, D, L, B, T, ws.winv, ws.logpart, ws.epoch, ws.info, ws.err, i, n, b, a);
   hipLaunchKernelGGL(k_panel_flow, dim3(1 + fp.helpers), dim3(BTA_THREADS), 0, st, p);
+}
+
+// Batched form: entry e of a launch owns workgroups [begin[e], begin[e+1]):
+// its chain first, then its helpers.  All entries' workgroups are resident
+// together (their sum is within the device's capacity).
+__global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow_batched(const FlowArgs* __restrict__ args,
+                                                                       const int* __restrict__ begin, int nent) {
+  __shared__ double sA[64 * BTA_LDS_LD];
+  __shared__ double sB[64 * BTA_LDS_LD];
+  __shared__ double sScr[4 * 64 + 64];
+  __shared__ int ok_sh;
+  int lo = 0, hi = nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const FlowArgs p = args[lo];
+  const int local = (int)blockIdx.x - begin[lo];
+  const unsigned long long tag = (p.epoch[0] << 20) | (unsigned long long)p.blk;
+  if (local == 0) flow_chain(p, tag, sA, sB, sScr, &ok_sh);
+  else flow_helper(p, local - 1, tag, sA, sB, &ok_sh);
+}
+
+void flow_launch_batched(const FlowArgs* d_args, const int* d_begin, int nent, int total, hipStream_t st) {
+  hipLaunchKernelGGL(k_panel_flow_batched, dim3(total), dim3(BTA_THREADS), 0, st, d_args, d_begin, nent);
+}
+
+int flow_capacity(int device) {
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_panel_flow, BTA_THREADS, 0) != hipSuccess || per <= 0)
+    return 0;
+  int per_b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_b, k_panel_flow_batched, BTA_THREADS, 0) == hipSuccess &&
+      per_b > 0)
+    per = std::min(per, per_b);
+  return cus * per;
 }
 
 }  // namespace bta

@@ -559,10 +559,69 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
 BatchPlan::~BatchPlan() {
   if (d_probs) hipFree(d_probs);
   if (d_ent) hipFree(d_ent);
+  if (d_fargs) hipFree(d_fargs);
+  if (d_fbegin) hipFree(d_fbegin);
 }
 
-int BatchPlan::build(const std::vector<ProbDesc>& ps) {
+int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int device) {
   probs = ps;
+  // Persistent dataflow form when every problem has more than one tile column
+  // (or when asked for): each problem gets a chain and an equal share of the
+  // device's resident workgroups as helpers, all in one launch per block step.
+  bool all_wide = true;
+  for (const auto& d : ps) all_wide = all_wide && d.b > 64;
+  flow = (panel_mode == 2) || (panel_mode == 0 && all_wide);
+  if (flow) {
+    const int cap = flow_capacity(device);
+    const int np = (int)ps.size();
+    const int helpers = std::min(1023, cap / np - 1);
+    if (helpers < 1) {
+      flow = false;  // too many problems to give each a resident chain + helper
+    } else {
+      std::vector<FlowArgs> args;
+      std::vector<int> begin;
+      int nmax = 0;
+      for (const auto& d : ps) nmax = std::max(nmax, d.n);
+      fplans.clear();
+      for (const auto& d : ps) {
+        fplans.emplace_back(new FlowPlan());
+        const int rc = fplans.back()->build(d.nT, d.a > 0 ? d.nTa : 0, helpers, 2);
+        if (rc) return rc;
+        fplans.back()->spin_ticks = flow_spin_ticks(device, 20.0);
+      }
+      for (int i = 0; i < nmax; ++i) {
+        FLaunch L;
+        L.off_args = (int)args.size();
+        L.off_begin = (int)begin.size();
+        int total = 0;
+        for (int q = 0; q < np; ++q) {
+          const ProbDesc& d = ps[q];
+          if (i >= d.n) continue;
+          args.push_back(flow_args(*fplans[q], d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch, d.info, d.err, i,
+                                   d.n, d.b, d.a));
+          begin.push_back(total);
+          total += 1 + helpers;
+        }
+        begin.push_back(total);
+        L.nent = (int)args.size() - L.off_args;
+        L.total = total;
+        flaunches.push_back(L);
+      }
+      hipError_t e;
+      if ((e = hipMalloc(&d_fargs, args.size() * sizeof(FlowArgs))) != hipSuccess) return (int)e;
+      if ((e = hipMemcpy(d_fargs, args.data(), args.size() * sizeof(FlowArgs), hipMemcpyHostToDevice)) !=
+          hipSuccess)
+        return (int)e;
+      if ((e = hipMalloc(&d_fbegin, begin.size() * sizeof(int))) != hipSuccess) return (int)e;
+      if ((e = hipMemcpy(d_fbegin, begin.data(), begin.size() * sizeof(int), hipMemcpyHostToDevice)) !=
+          hipSuccess)
+        return (int)e;
+      if ((e = hipMalloc(&d_probs, ps.size() * sizeof(ProbDesc))) != hipSuccess) return (int)e;
+      if ((e = hipMemcpy(d_probs, ps.data(), ps.size() * sizeof(ProbDesc), hipMemcpyHostToDevice)) != hipSuccess)
+        return (int)e;
+      return 0;
+    }
+  }
   int nmax = 0;
   for (const auto& d : ps) nmax = std::max(nmax, d.n);
   std::vector<int> ent;  // per launch: [prob..., begin..., total]
@@ -639,6 +698,13 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps) {
 int BatchPlan::enqueue(hipStream_t st) const {
   const int np = (int)probs.size();
   hipLaunchKernelGGL(k_begin_batch, dim3((np + 63) / 64), dim3(64), 0, st, d_probs, np);
+  if (flow) {  // block updates are folded into the panel launches
+    for (const FLaunch& L : flaunches)
+      flow_launch_batched(d_fargs + L.off_args, d_fbegin + L.off_begin, L.nent, L.total, st);
+    hipLaunchKernelGGL(k_tip_potrf_batched, dim3(np), dim3(BTA_THREADS), 0, st, d_probs);
+    hipLaunchKernelGGL(k_logdet_reduce_batched, dim3(np), dim3(BTA_THREADS), 0, st, d_probs);
+    return (int)hipGetLastError();
+  }
   for (const Launch& L : launches) {
     if (L.total > 0)
       hipLaunchKernelGGL(k_panel_step_batched, dim3(L.total), dim3(BTA_THREADS), 0, st, d_probs,

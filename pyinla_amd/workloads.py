@@ -92,23 +92,32 @@ def known_factor_bta(n: int = 50, b: int = 92, a: int = 4, seed: int = 0) -> Kno
     return KnownFactorBTA(n, b, a, Ld, Ll, Lb, Lt, QD, QL, QB, QT, rhs)
 
 
-def known_factor_bta_torch(n: int, b: int, a: int, seed: int, device):
+def known_factor_bta_torch(n: int, b: int, a: int, seed: int, device, norm_b: int | None = None):
     """Large known-factor BTA matrix generated on ``device`` with torch.
 
     Used where numpy generation would take minutes (b in the thousands).  Same
     distributions as :func:`known_factor_draws` but drawn from a torch
     generator seeded with ``seed`` on ``device``; returns (factor, Q) tuples of
     float64 tensors (D, L, B, T).
+
+    ``norm_b``: if given, the N(0, 0.1) / N(0, 0.05) off-diagonal draws are
+    scaled by sqrt(norm_b / b), so that the block spectral norms equal those of
+    the b = norm_b construction.  Unscaled, the sub-diagonal factor blocks
+    have norm ~0.2 sqrt(b) (9 at b = 2000): the block recurrence of L^-1 then
+    amplifies by that factor per block and Q = L L^T is numerically singular
+    in FP64 for n >= 3 -- every Cholesky of it fails, the reference's
+    included.
     """
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(seed)
     f64 = torch.float64
-    Ld = torch.randn((n, b, b), generator=g, device=device, dtype=f64).mul_(0.1).tril_(-1)
+    sc = (norm_b / b) ** 0.5 if norm_b else 1.0
+    Ld = torch.randn((n, b, b), generator=g, device=device, dtype=f64).mul_(0.1 * sc).tril_(-1)
     dg = torch.randn((n, b), generator=g, device=device, dtype=f64).abs_().add_(1.0)
     Ld.diagonal(dim1=1, dim2=2).copy_(dg)
-    Ll = torch.randn((max(n - 1, 0), b, b), generator=g, device=device, dtype=f64).mul_(0.1)
-    Lb = torch.randn((n, a, b), generator=g, device=device, dtype=f64).mul_(0.05)
+    Ll = torch.randn((max(n - 1, 0), b, b), generator=g, device=device, dtype=f64).mul_(0.1 * sc)
+    Lb = torch.randn((n, a, b), generator=g, device=device, dtype=f64).mul_(0.05 * sc)
     Lt = torch.randn((a, a), generator=g, device=device, dtype=f64).mul_(0.05).tril_(-1)
     Lt.diagonal().copy_(torch.randn((a,), generator=g, device=device, dtype=f64).abs_().add_(1.0))
     Q = _q_from_factor_torch(Ld, Ll, Lb, Lt)

@@ -140,3 +140,46 @@ def test_autotuner_keeps_results_exact(cuda):
     ob.pobtasi(D, L, B, T)
     assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
     assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
+
+
+@pytest.mark.parametrize("force_flow", [False, True])
+def test_batched_flow_factorization(cuda, force_flow):
+    """All problems wider than one tile: the batch runs as persistent dataflow
+    launches (one chain + an equal share of helpers per problem); with
+    force_flow the single-tile-column problems take that form too."""
+    from pyinla_amd.batch import BatchedFactorization
+    shapes = [(3, 200, 6, "bta"), (2, 130, 0, "bt"), (4, 92, 4, "bta"), (1, 300, 5, "bta"), (3, 150, 3, "bt")]
+    if force_flow:
+        shapes.append((3, 60, 2, "bta"))
+    kfs, solvers, sps_ = [], [], []
+    for q, (n, b, a, sp) in enumerate(shapes):
+        kf = known_factor_bta(n, b, a, seed=500 + q)
+        s = SerinvSolver(None, b, n, a)
+        s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if a else None, kf.Q_T if a else None)
+        kfs.append(kf)
+        solvers.append(s)
+        sps_.append(sp)
+    if force_flow:
+        solvers[0].set_panel("flow")
+    print(f"[batch-flow] force={force_flow} shapes={shapes}", flush=True)
+    batch = BatchedFactorization(solvers, sps_)
+    for rep in range(2):
+        if rep:
+            for kf, s in zip(kfs, solvers):
+                s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if kf.a else None, kf.Q_T if kf.a else None)
+        batch.factorize()
+        for (n, b, a, sp), kf, s in zip(shapes, kfs, solvers):
+            D, L = kf.Q_D.copy(), kf.Q_L.copy()
+            B = kf.Q_B.copy() if sp == "bta" and a else None
+            T = kf.Q_T.copy() if sp == "bta" and a else None
+            ob.pobtaf(D, L, B, T)
+            info, ld = s.status()
+            assert info == 0
+            ld_o = ob.logdet(D, T)
+            assert abs(ld - ld_o) <= 1e-10 * abs(ld_o)
+            assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
+            if n > 1:
+                assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < RTOL
+            if B is not None:
+                assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
+                assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
