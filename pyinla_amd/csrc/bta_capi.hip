@@ -165,6 +165,7 @@ int bta_ws_destroy(bta_ws_t ws) {
   ws->w.potri_plan.reset();
   ws->w.update_plan.reset();
   hipFree(ws->w.winv);
+  if (ws->w.gfull) hipFree(ws->w.gfull);
   hipFree(ws->w.flags);
   if (ws->w.wfull) hipFree(ws->w.wfull);
   delete ws;

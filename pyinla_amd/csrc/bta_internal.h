@@ -79,6 +79,9 @@ struct Workspace {
   // Full inverses W_i = D_i^-1 of the diagonal factor blocks (n * b * b),
   // allocated on first use by the solve / selected inversion.
   double* wfull = nullptr;
+  // G_i = L_i W_i (n b^2) and Ga_i = B_i W_i (n a b) of the selected inversion,
+  // allocated on its first use.
+  double* gfull = nullptr;
   bool w_valid = false;            // W matches the factor currently held
   const double* w_of = nullptr;    // D pointer W was computed from
   // cached launch plans, keyed by the block pointers they were built for

@@ -10,11 +10,14 @@
 // blocks, so the whole inverse costs ~2 log2(b/64) launches.
 //
 // Selected inversion then runs the Takahashi recurrence backwards in i with
-// G = L_i W_i and Ga = B_i W_i (three grouped launches per block):
-//     Sigma_{n,i}   = -Sigma_{n,i+1} G - Sigma_T Ga
-//     Sigma_{i+1,i} = -Sigma_{i+1,i+1} G - Sigma_{n,i+1}^T Ga
-//     Sigma_ii      =  W_i^T W_i - G^T Sigma_{i+1,i} - Ga^T Sigma_{n,i}
-// with Sigma_T = T^-T T^-1.  Sigma_ii is written full symmetric.
+// G_i = L_i W_i and Ga_i = B_i W_i:
+//     Sigma_{n,i}   = -Sigma_{n,i+1} G_i - Sigma_T Ga_i
+//     Sigma_{i+1,i} = -Sigma_{i+1,i+1} G_i - Sigma_{n,i+1}^T Ga_i
+//     Sigma_ii      =  W_i^T W_i - G_i^T Sigma_{i+1,i} - Ga_i^T Sigma_{n,i}
+// with Sigma_T = T^-T T^-1.  G_i, Ga_i and W_i^T W_i do not depend on the
+// recurrence: one launch computes them for every block at once (full
+// occupancy), leaving two dependent launches per block.  Sigma_ii is written
+// full symmetric.
 #include "bta_internal.h"
 
 namespace bta {
@@ -101,52 +104,78 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
   bool same = ws.potri_plan && ws.potri_a == a && ws.potri_tile == ws.tile_grouped;
   for (int q = 0; q < 4 && same; ++q) same = (ws.potri_key[q] == key[q]);
   if (same) return 0;
-  double* G = ws.work;                      // b x b
-  double* Ga = ws.work + (size_t)b * b;     // a x b
+  // G_i = L_i W_i and Ga_i = B_i W_i for every block (they depend on the
+  // factor only), kept for the whole recurrence
+  if (!ws.gfull) {
+    const size_t nd = (size_t)n * b * b + (size_t)n * std::max(a, 1) * b;
+    hipError_t e = hipMalloc(&ws.gfull, nd * sizeof(double));
+    if (e != hipSuccess) return (int)e;
+  }
+  double* Gall = ws.gfull;
+  double* Gaall = ws.gfull + (size_t)n * b * b;
   std::vector<GemmLaunch> ls;
   const int ts = choose_tile(ws.tile_grouped, b, b);
-  if (a > 0) {  // Sigma_T = Tinv^T Tinv
-    GemmLaunch l0;
-    GemmJob j = job(T, a, a, a, 0.0);
-    add_seg(j, seg(ws.tinv, 64, 1, ws.tinv, 64, 0, a, 1.0));
-    l0.add(j);
+  // One launch of everything the recurrence does not depend on:
+  //   G_i = L_i W_i, Ga_i = B_i W_i, Sigma_ii <- W_i^T W_i (the first term of
+  //   Sigma_ii; D_i is free once W_i exists), Sigma_T = T^-T T^-1.
+  {
+    GemmLaunch l0(ts);
+    for (int i = 0; i < n; ++i) {
+      double* Di = D + (int64_t)i * b * b;
+      double* Wi = ws.wfull + (int64_t)i * b * b;
+      if (i < n - 1) {
+        GemmJob j = job(Gall + (int64_t)i * b * b, b, b, b, 0.0);
+        add_seg(j, seg(L + (int64_t)i * b * b, b, 0, Wi, b, 0, b, 1.0, TRI_B_LOWER));
+        l0.add(j);
+      }
+      if (a > 0) {
+        GemmJob j = job(Gaall + (int64_t)i * a * b, a, b, b, 0.0);
+        add_seg(j, seg(B + (int64_t)i * a * b, b, 0, Wi, b, 0, b, 1.0, TRI_B_LOWER));
+        l0.add(j);
+      }
+      GemmJob sd = job(Di, b, b, b, 0.0, /*lower=*/1, /*mirror=*/1);
+      add_seg(sd, seg(Wi, b, 1, Wi, b, 0, b, 1.0, TRI_A_UPPER | TRI_B_LOWER));
+      l0.add(sd);
+    }
+    if (a > 0) {
+      GemmJob j = job(T, a, a, a, 0.0);
+      add_seg(j, seg(ws.tinv, 64, 1, ws.tinv, 64, 0, a, 1.0));
+      l0.add(j);
+    }
     ls.push_back(std::move(l0));
   }
+  // The recurrence, two dependent launches per block (i = n-1 .. 0):
+  //   Sigma_{i+1,i} = -Sigma_{i+1,i+1} G_i - Sigma_{n,i+1}^T Ga_i   (into L_i)
+  //   Sigma_{n,i}   = -Sigma_{n,i+1} G_i - Sigma_T Ga_i            (into B_i)
+  //   Sigma_ii     +=  - G_i^T Sigma_{i+1,i} - Ga_i^T Sigma_{n,i}  (into D_i)
   for (int i = n - 1; i >= 0; --i) {
     const bool hasL = i < n - 1;
     double* Di = D + (int64_t)i * b * b;
-    double* Wi = ws.wfull + (int64_t)i * b * b;
     double* Li = hasL ? L + (int64_t)i * b * b : nullptr;
     double* Bi = a > 0 ? B + (int64_t)i * a * b : nullptr;
     double* Dn = hasL ? D + (int64_t)(i + 1) * b * b : nullptr;
     double* Bn = (hasL && a > 0) ? B + (int64_t)(i + 1) * a * b : nullptr;
-    GemmLaunch l1(ts), l2(ts), l3(ts);
+    double* G = hasL ? Gall + (int64_t)i * b * b : nullptr;
+    double* Ga = a > 0 ? Gaall + (int64_t)i * a * b : nullptr;
+    GemmLaunch l2(ts), l3(ts);
     if (hasL) {
-      GemmJob j = job(G, b, b, b, 0.0);
-      add_seg(j, seg(Li, b, 0, Wi, b, 0, b, 1.0, TRI_B_LOWER));
-      l1.add(j);
+      GemmJob s2 = job(Li, b, b, b, 0.0);  // Sigma_{i+1,i}
+      add_seg(s2, seg(Dn, b, 0, G, b, 0, b, -1.0));
+      if (a > 0) add_seg(s2, seg(Bn, b, 1, Ga, b, 0, a, -1.0));
+      l2.add(s2);
     }
     if (a > 0) {
-      GemmJob j = job(Ga, a, b, b, 0.0);
-      add_seg(j, seg(Bi, b, 0, Wi, b, 0, b, 1.0, TRI_B_LOWER));
-      l1.add(j);
-      GemmJob s = job(Bi, a, b, b, 0.0);  // Sigma_{n,i}
-      if (hasL) add_seg(s, seg(Bn, b, 0, G, b, 0, b, -1.0));
-      add_seg(s, seg(T, a, 0, Ga, b, 0, a, -1.0));
-      l2.add(s);
+      GemmJob s2 = job(Bi, a, b, b, 0.0);  // Sigma_{n,i}
+      if (hasL) add_seg(s2, seg(Bn, b, 0, G, b, 0, b, -1.0));
+      add_seg(s2, seg(T, a, 0, Ga, b, 0, a, -1.0));
+      l2.add(s2);
     }
-    if (hasL) {
-      GemmJob s = job(Li, b, b, b, 0.0);  // Sigma_{i+1,i}
-      add_seg(s, seg(Dn, b, 0, G, b, 0, b, -1.0));
-      if (a > 0) add_seg(s, seg(Bn, b, 1, Ga, b, 0, a, -1.0));
-      l2.add(s);
+    if (hasL || a > 0) {
+      GemmJob sd = job(Di, b, b, b, 1.0, /*lower=*/1, /*mirror=*/1);  // Sigma_ii
+      if (hasL) add_seg(sd, seg(G, b, 1, Li, b, 0, b, -1.0));
+      if (a > 0) add_seg(sd, seg(Ga, b, 1, Bi, b, 0, a, -1.0));
+      l3.add(sd);
     }
-    GemmJob sd = job(Di, b, b, b, 0.0, /*lower=*/1, /*mirror=*/1);  // Sigma_ii
-    add_seg(sd, seg(Wi, b, 1, Wi, b, 0, b, 1.0, TRI_A_UPPER | TRI_B_LOWER));
-    if (hasL) add_seg(sd, seg(G, b, 1, Li, b, 0, b, -1.0));
-    if (a > 0) add_seg(sd, seg(Ga, b, 1, Bi, b, 0, a, -1.0));
-    l3.add(sd);
-    ls.push_back(std::move(l1));
     ls.push_back(std::move(l2));
     ls.push_back(std::move(l3));
   }

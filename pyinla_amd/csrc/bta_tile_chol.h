@@ -1,7 +1,7 @@
 // In-workgroup Cholesky factorisation and inverse of one 64x64 FP64 tile.
 //
-// Right-looking over 16 panels of 4 columns, grouped in four 16-column tile
-// columns; the matrix A (-> L) and E (-> L^-1, started at I) live in LDS.
+// Right-looking over 16 panels of 4 columns (four per 16-column tile
+// column); the matrix A (-> L) and E (-> L^-1, started at I) live in LDS.
 // Panel p = [p, p+4):
 //   phase A (all 256 threads, no cross-wave dependence):
 //     every thread factors the 4x4 diagonal block A_pp = l l^T itself and
@@ -10,13 +10,13 @@
 //     element each; the other 4 (p + 4) threads replace the panel rows of E
 //     by inv(l) E_p (columns <= p + 3, the only non-zero ones);
 //   phase B (after one barrier):
-//     rank-4 updates with v_mfma_f64_16x16x4f64 (K = 4 is the panel width):
-//     the rest of the current 16-column tile column, A -= X X^T, and the
-//     rows of E below the panel, E_r -= X E_p;
-// and after the four panels of a tile column, the tiles to its right receive
-// the rank-16 update in one pass.  Row r of E ends as row r of L^-1 because
-// every panel step applies the row operations of the forward substitution
-// L W = I to it.  Two barriers per panel, 38 in all.
+//     rank-4 updates with v_mfma_f64_16x16x4f64 (K = 4 is the panel width)
+//     of every 16x16 tile of the trailing lower matrix, A -= X X^T, and of
+//     the rows of E below the panel, E_r -= X E_p -- at most 14 tile jobs,
+//     dealt round-robin over the four waves.
+// Row r of E ends as row r of L^-1 because every panel step applies the row
+// operations of the forward substitution L W = I to it.  Two barriers per
+// panel, 33 in all.
 #pragma once
 #include "bta_device.h"
 
@@ -185,44 +185,54 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
         else v = (j == 0) ? l30 : (j == 1 ? l31 : (j == 2 ? l32 : l33));
         s[(p + i) * BTA_LDS_LD + p + j] = v;
       }
-      if (wv >= ts) {
+      {
+        // Jobs of this panel, dealt round-robin over the 4 waves (<= 4 each):
+        //   A: every lower tile (ri, ci), ci >= ts, of the trailing matrix,
+        //      C -= X_ri X_ci^T restricted to rows and columns > p + 3;
+        //   E: tiles (ri, ci), ri >= ts, ci <= ts, E_r -= X_r E_p (rows > p + 3).
         const int li = lane & 15, lk = lane >> 4;
-        const int ra = 16 * wv + li;  // row of the A operand (this wave's tile row)
-        const int rb = 16 * ts + li;  // row of the B operand (tile column ts)
-        const double xa = (ra > p + 3) ? s[ra * BTA_LDS_LD + p + lk] : 0.0;
-        const double xb = (rb > p + 3) ? s[rb * BTA_LDS_LD + p + lk] : 0.0;
-        f64x4 acc = mfma_f64(-xa, xb, load16(s, 16 * wv, 16 * ts));
-        // rows of E in this tile row below the panel: E_r -= X_r E_p, tiles (wv, 0..ts)
-        f64x4 eacc[4];
+        const int nA = (4 - ts) * (5 - ts) / 2, nE = (4 - ts) * (ts + 1);
+        f64x4 acc[4];
+        int jr[4], jc[4], jk[4];
 #pragma unroll
-        for (int ci = 0; ci < 4; ++ci) {
-          if (ci <= ts) {
-            const double eb = w[(p + lk) * BTA_LDS_LD + 16 * ci + li];
-            eacc[ci] = mfma_f64(-xa, eb, load16(w, 16 * wv, 16 * ci));
+        for (int q = 0; q < 4; ++q) {
+          const int j = wv + 4 * q;
+          jk[q] = (j < nA) ? 0 : (j < nA + nE ? 1 : 2);
+          int ri = 0, ci = 0;
+          if (jk[q] == 0) {
+            int t = j;
+            ci = ts;
+            while (t >= 4 - ci) {
+              t -= 4 - ci;
+              ++ci;
+            }
+            ri = ci + t;
+          } else if (jk[q] == 1) {
+            const int t = j - nA;
+            ri = ts + t / (ts + 1);
+            ci = t % (ts + 1);
+          }
+          jr[q] = ri;
+          jc[q] = ci;
+          if (jk[q] < 2) {
+            const int ra = 16 * ri + li;
+            const double xa = (ra > p + 3) ? s[ra * BTA_LDS_LD + p + lk] : 0.0;
+            double xb;
+            if (jk[q] == 0) {
+              const int rb = 16 * ci + li;
+              xb = (rb > p + 3) ? s[rb * BTA_LDS_LD + p + lk] : 0.0;
+              acc[q] = mfma_f64(-xa, xb, load16(s, 16 * ri, 16 * ci));
+            } else {
+              xb = w[(p + lk) * BTA_LDS_LD + 16 * ci + li];
+              acc[q] = mfma_f64(-xa, xb, load16(w, 16 * ri, 16 * ci));
+            }
           }
         }
-        store16_below(s, 16 * wv, 16 * ts, acc, p + 3, p + 3);
 #pragma unroll
-        for (int ci = 0; ci < 4; ++ci)
-          if (ci <= ts) store16_below(w, 16 * wv, 16 * ci, eacc[ci], p + 3, -1);
-      }
-      __syncthreads();
-    }
-    // tiles right of tile column ts: A[ri][ci] -= X[ri] X[ci]^T (K = 16)
-    const int nr = 3 - ts;
-    if (nr > 0) {
-      const int ntr = nr * (nr + 1) / 2;
-#pragma unroll 1
-      for (int t = wv; t < ntr; t += 4) {
-        int rr = 0, cc = t;
-        while (cc > rr) {
-          cc -= rr + 1;
-          ++rr;
+        for (int q = 0; q < 4; ++q) {
+          if (jk[q] == 0) store16_below(s, 16 * jr[q], 16 * jc[q], acc[q], p + 3, p + 3);
+          else if (jk[q] == 1) store16_below(w, 16 * jr[q], 16 * jc[q], acc[q], p + 3, -1);
         }
-        const int ri = ts + 1 + rr, ci = ts + 1 + cc;
-        f64x4 acc = load16(s, 16 * ri, 16 * ci);
-        acc = mma16<true>(s, 16 * ri, 16 * ts, s, 16 * ts, 16 * ci, 16, -1.0, acc);
-        store16(s, 16 * ri, 16 * ci, acc);
       }
       __syncthreads();
     }
