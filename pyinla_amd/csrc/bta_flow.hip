@@ -220,9 +220,11 @@ __device__ __forceinline__ void flow_trsm(TileAcc& x, const double* sC, const do
   mma_nt<false>(x, sC, sW, 0, kend);
 }
 
-// Chain: publish the factored diagonal tile k held in sA (L_kk) / sB (inverse).
-__device__ __forceinline__ void chain_publish_diag(const FlowArgs& p, int k, const TileCholResult& res,
-                                                   const double* sA, const double* sB, unsigned long long tag) {
+// Chain: store the factored diagonal tile k held in sA (L_kk) / sB (inverse).
+// The flag is published separately (chain_flag), so that the write-through
+// stores drain while the chain already computes.
+__device__ __forceinline__ void chain_store_diag(const FlowArgs& p, int k, const TileCholResult& res,
+                                                 const double* sA, const double* sB) {
   int rows, cols;
   double* g = flow_tile(p, k, k, rows, cols);
   double* w = p.winv + (int64_t)k * 4096;
@@ -235,59 +237,61 @@ __device__ __forceinline__ void chain_publish_diag(const FlowArgs& p, int k, con
     p.logpart[k] = res.logsum;
     if (res.fail >= 0) atomicMin(p.info, (unsigned long long)(p.row0 + (long long)k * 64 + res.fail));
   }
-  flow_publish(wflag(p, k), tag);
 }
 
+// Per column k the chain
+//   1. waits for the hand-off tiles (k+1, k) and (k+1, k+1) (updated by their
+//      helpers through column k-1) and loads both;
+//   2. solves X = C(k+1, k) inv(L_kk)^T and issues its stores;
+//   3. publishes inv(L_kk) (stored one step earlier: the stores have drained);
+//   4. applies C(k+1, k+1) -= X X^T, then publishes X (its stores drained
+//      during the update);
+//   5. factors C(k+1, k+1) and issues the stores of L and its inverse.
+// Deferring each flag past the next piece of work delays its consumers by a
+// few microseconds; they are needed one or two columns later.
 __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA, double* sB, double* sScr,
                            int* ok_sh) {
   const int nT = p.nT;
   int rows, cols;
-  TileAcc acc;
-  // F(0): the first diagonal tile has no in-panel updates
+  TileAcc acc, accd;
   {
     const double* g = flow_tile(p, 0, 0, rows, cols);
     acc_load(acc, g, p.b, rows, cols);
     acc_to_lds(acc, sA);
     __syncthreads();
     const TileCholResult res = tile_chol_inv(sA, sB, sScr, rows);
-    chain_publish_diag(p, 0, res, sA, sB, tag);
+    chain_store_diag(p, 0, res, sA, sB);
+    if (nT == 1) flow_publish(wflag(p, 0), tag);
   }
   for (int k = 0; k + 1 < nT; ++k) {
-    // X(k+1, k) = C(k+1, k) inv(L_kk)^T ; C updated by its helper through column k-1
     if (k == 0) {
-      const double* g = flow_tile(p, 1, 0, rows, cols);
-      acc_load(acc, g, p.b, rows, cols);
+      acc_load(acc, flow_tile(p, 1, 0, rows, cols), p.b, rows, cols);
+      acc_load(accd, flow_tile(p, 1, 1, rows, cols), p.b, rows, cols);
     } else {
       if (!flow_wait(p, hflag(p, k, 0), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 2);
+      if (!flow_wait(p, hflag(p, k, 1), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 2);
       acc_load_sys(acc, hsub(p, k));
+      acc_load_sys(accd, hdiag(p, k + 1));
     }
     acc_to_lds(acc, sA);
     __syncthreads();
-    TileAcc x;
-    flow_trsm(x, sA, sB);
+    flow_trsm(acc, sA, sB);  // X(k+1, k) in acc
     __syncthreads();
     {
       double* g = flow_tile(p, k + 1, k, rows, cols);
-      acc_store(x, g, p.b, rows, cols);
-      acc_store_sys(x, xtile(p, k + 1, k));
-      acc_to_lds(x, sA);
+      acc_store(acc, g, p.b, rows, cols);
+      acc_store_sys(acc, xtile(p, k + 1, k));
+      acc_to_lds(acc, sA);
     }
+    flow_publish(wflag(p, k), tag);  // inv(L_kk), stored during the previous step
+    mma_nt<true>(accd, sA, sA, 0, 64);
     flow_publish(xflag(p, k + 1, k), tag);
-    // C(k+1, k+1) -= X X^T, then factor it
-    if (k == 0) {
-      const double* g = flow_tile(p, 1, 1, rows, cols);
-      acc_load(acc, g, p.b, rows, cols);
-    } else {
-      if (!flow_wait(p, hflag(p, k, 1), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 2);
-      acc_load_sys(acc, hdiag(p, k + 1));
-    }
-    mma_nt<true>(acc, sA, sA, 0, 64);
-    __syncthreads();
-    acc_to_lds(acc, sA);
+    acc_to_lds(accd, sA);
     __syncthreads();
     const int mv = min(64, p.b - 64 * (k + 1));
     const TileCholResult res = tile_chol_inv(sA, sB, sScr, mv);
-    chain_publish_diag(p, k + 1, res, sA, sB, tag);
+    chain_store_diag(p, k + 1, res, sA, sB);
+    if (k + 2 == nT) flow_publish(wflag(p, k + 1), tag);
   }
 }
 
