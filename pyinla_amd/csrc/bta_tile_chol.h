@@ -74,15 +74,16 @@ __device__ __forceinline__ void store16(double* __restrict__ C, int r0, int c0, 
   for (int r = 0; r < 4; ++r) C[(r0 + 4 * r + (lane >> 4)) * BTA_LDS_LD + c0 + (lane & 15)] = v[r];
 }
 
-// Store only the elements with global row > rmin and global column > cmin.
+// Store only the elements with global row > rmin and global column > cmin
+// (and, with lower, on or below the diagonal).
 __device__ __forceinline__ void store16_below(double* __restrict__ C, int r0, int c0, f64x4 v, int rmin,
-                                              int cmin) {
+                                              int cmin, bool lower) {
   const int lane = threadIdx.x & 63;
   const int j = c0 + (lane & 15);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int i = r0 + 4 * r + (lane >> 4);
-    if (i > rmin && j > cmin) C[i * BTA_LDS_LD + j] = v[r];
+    if (i > rmin && j > cmin && (!lower || j <= i)) C[i * BTA_LDS_LD + j] = v[r];
   }
 }
 
@@ -98,9 +99,11 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
   double* piv = scratch;  // [64] pivots (squared diagonal of L)
   int fail = -1;
 
+  // identity outside the valid block, zero strict upper triangle (the
+  // updates below never store above the diagonal), E = I
   for (int e = tid; e < 64 * 64; e += BTA_THREADS) {
     const int i = e >> 6, j = e & 63;
-    if (i >= mv || j >= mv) s[i * BTA_LDS_LD + j] = (i == j) ? 1.0 : 0.0;
+    if (i >= mv || j >= mv || j > i) s[i * BTA_LDS_LD + j] = (i == j) ? 1.0 : 0.0;
     w[i * BTA_LDS_LD + j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -230,17 +233,12 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          if (jk[q] == 0) store16_below(s, 16 * jr[q], 16 * jc[q], acc[q], p + 3, p + 3);
-          else if (jk[q] == 1) store16_below(w, 16 * jr[q], 16 * jc[q], acc[q], p + 3, -1);
+          if (jk[q] == 0) store16_below(s, 16 * jr[q], 16 * jc[q], acc[q], p + 3, p + 3, jr[q] == jc[q]);
+          else if (jk[q] == 1) store16_below(w, 16 * jr[q], 16 * jc[q], acc[q], p + 3, -1, false);
         }
       }
       __syncthreads();
     }
-  }
-  // the rank updates also wrote the strict upper part of diagonal tiles
-  for (int e = tid; e < 64 * 64; e += BTA_THREADS) {
-    const int i = e >> 6, j = e & 63;
-    if (j > i) s[i * BTA_LDS_LD + j] = 0.0;
   }
   // sum of log pivots: fixed butterfly over wave 0 (deterministic order)
   TileCholResult res;
