@@ -17,12 +17,14 @@ from __future__ import annotations
 
 import argparse
 import json
+from pathlib import Path
 import os
 import sys
 import time
 
 import numpy as np
 
+ROOT = Path(__file__).resolve().parent
 METRIC = "f(θ)+∇f evals/s, adjoint vs FD; pobtaf/pobtasi FP64 TFLOP/s vs peak"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -320,7 +322,12 @@ def main():
         _lib.check(lib.bta_ws_set_timing(s._ws, 0), "set_timing")
     fp_c, fu_c = flops_potrf_split(n, b, a)
     fp_p, fu_p = flops_potrf_split(n, b, 0)
-    flops_cls = {0: fp_c + fp_p, 1: fu_c + fu_p, 2: flops_potri(n, b, a) + flops_potri(n, b, 0)}
+    flow = cnt[1] == 0  # persistent panel launches carry the block updates too
+    if flow:
+        flops_cls = {0: flops_potrf(n, b, a) + flops_potrf(n, b, 0), 1: 0.0}
+    else:
+        flops_cls = {0: fp_c + fp_p, 1: fu_c + fu_p}
+    flops_cls[2] = flops_potri(n, b, a) + flops_potri(n, b, 0)
     potrf_ms = tot[0] + tot[1]
     potrf_tflops = (flops_potrf(n, b, a) + flops_potrf(n, b, 0)) / (potrf_ms * 1e-3) / 1e12 if potrf_ms else None
     potri_tflops = flops_cls[2] / (tot[2] * 1e-3) / 1e12 if tot[2] else None
@@ -335,7 +342,20 @@ def main():
         ach = per_launch_flops / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
-    roof["kernel"] = {0: "k_panel_step", 1: "k_gemm_nt_sub", 2: "k_gemm_grouped", 3: "k_gemv"}[dom]
+    roof["kernel"] = {0: "k_panel_flow" if flow else "k_panel_step", 1: "k_gemm_nt_sub", 2: "k_gemm_grouped",
+                      3: "k_gemv"}[dom]
+    # HBM bytes per launch of that kernel from the committed rocprofv3 PMC
+    # summary (FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy), if present
+    tr_path = ROOT / "profiles" / "pmc_traffic.json"
+    if tr_path.exists():
+        try:
+            tr = json.loads(tr_path.read_text())
+            k = tr.get("kernels", {}).get("bta::" + roof["kernel"])
+            if k:
+                roof["traffic"] = k["traffic_bytes"]
+                roof["traffic_source"] = f"profiles/pmc_traffic.json ({tr.get('workload', '')})"
+        except (ValueError, KeyError):
+            pass
     roof["launches"] = cnt[dom]
     roof["avg_launch_ms"] = tot[dom] / max(cnt[dom], 1)
     breakdown = {classes[c]: {"ms": tot[c], "launches": cnt[c]} for c in classes}
