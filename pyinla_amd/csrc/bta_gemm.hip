@@ -6,16 +6,27 @@
 
 namespace bta {
 
-// acc += alpha * sA(64 x K) * sB(64 x K)^T over k in [0, kend).
+// K chunk of the 64x64-tile kernel: 32 keeps the LDS footprint at 34 KB, so
+// four workgroups share a CU (latency hiding, and 1024 resident tiles)
+#ifndef BTA_GEMM_KC
+#define BTA_GEMM_KC 32
+#endif
+#define BTA_GEMM_LD (BTA_GEMM_KC + 1)  // odd stride: conflict-free 16-row fragment reads
+#ifndef BTA_GEMM_MINB
+#define BTA_GEMM_MINB 4  // resident workgroups per CU the register budget is cut for
+#endif
+
+// acc += alpha * sA(64 x K) * sB(64 x K)^T over k in [0, kend) (LDS stride LD).
+template <int LD = BTA_LDS_LD>
 __device__ __forceinline__ void mma_nt_alpha(TileAcc& t, const double* __restrict__ sA,
                                              const double* __restrict__ sB, int kend, double alpha) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
-  const double* a0p = sA + (wm + li) * BTA_LDS_LD + lk;
-  const double* a1p = sA + (wm + 16 + li) * BTA_LDS_LD + lk;
-  const double* b0p = sB + (wn + li) * BTA_LDS_LD + lk;
-  const double* b1p = sB + (wn + 16 + li) * BTA_LDS_LD + lk;
+  const double* a0p = sA + (wm + li) * LD + lk;
+  const double* a1p = sA + (wm + 16 + li) * LD + lk;
+  const double* b0p = sB + (wn + li) * LD + lk;
+  const double* b1p = sB + (wn + 16 + li) * LD + lk;
 #pragma unroll 4
   for (int k = 0; k < kend; k += 4) {
     const double a0 = alpha * a0p[k], a1 = alpha * a1p[k], b0 = b0p[k], b1 = b1p[k];
@@ -44,10 +55,39 @@ __device__ __forceinline__ void decode_lower(int idx, int& tr, int& tc) {
   tc = idx - r * (r + 1) / 2;
 }
 
-__global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __restrict__ jobs,
+// A 64 x BTA_GEMM_KC operand chunk staged global -> registers -> LDS (same
+// conventions as TilePrefetch: LDS layout s[i][k] of op(X)).
+struct ChunkPrefetch {
+  static constexpr int N = 64 * BTA_GEMM_KC / BTA_THREADS;
+  double r[N];
+  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw,
+                                       int mask, int i0, int k0) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const int e = threadIdx.x + BTA_THREADS * q;
+      if (trans == 0) {
+        const int i = e / BTA_GEMM_KC, k = e % BTA_GEMM_KC;
+        r[q] = (i < rows && k < kw && !masked_out(mask, i0 + i, k0 + k)) ? g[(int64_t)i * ld + k] : 0.0;
+      } else {
+        const int k = e >> 6, i = e & 63;
+        r[q] = (k < kw && i < rows && !masked_out(mask, i0 + i, k0 + k)) ? g[(int64_t)k * ld + i] : 0.0;
+      }
+    }
+  }
+  __device__ __forceinline__ void store(double* __restrict__ s, int trans) const {
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+      const int e = threadIdx.x + BTA_THREADS * q;
+      if (trans == 0) s[(e / BTA_GEMM_KC) * BTA_GEMM_LD + (e % BTA_GEMM_KC)] = r[q];
+      else s[(e & 63) * BTA_GEMM_LD + (e >> 6)] = r[q];
+    }
+  }
+};
+
+__global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(const GemmJob* __restrict__ jobs,
                                                              const int* __restrict__ prefix, int njobs) {
-  __shared__ double sA[64 * BTA_LDS_LD];
-  __shared__ double sB[64 * BTA_LDS_LD];
+  __shared__ double sA[64 * BTA_GEMM_LD];
+  __shared__ double sB[64 * BTA_GEMM_LD];
   const int jid = find_job(prefix, njobs, blockIdx.x);
   const GemmJob& g = jobs[jid];
   const int local = blockIdx.x - prefix[jid];
@@ -86,7 +126,7 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __r
   };
   // advance (seg_id, k0) to the next non-empty chunk; returns false at the end
   auto next_chunk = [&](bool first) -> bool {
-    if (!first) k0 += 64;
+    if (!first) k0 += BTA_GEMM_KC;
     while (seg_id < g.nseg) {
       if (first || k0 >= kmax) {
         if (!first) ++seg_id;
@@ -102,10 +142,10 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __r
     }
     return false;
   };
-  TilePrefetch pa, pb;
+  ChunkPrefetch pa, pb;
   auto fetch = [&]() {
     const GemmSeg& sg = g.seg[seg_id];
-    const int kw = min(64, kmax - k0);
+    const int kw = min(BTA_GEMM_KC, kmax - k0);
     const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
     const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
     if (sg.ta == 0) pa.load(sg.A + (int64_t)(tr * 64) * sg.lda + k0, sg.lda, 0, rows, kw, ma, tr * 64, k0);
@@ -117,14 +157,14 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped(const GemmJob* __r
   if (have) fetch();
   while (have) {
     const GemmSeg& sg = g.seg[seg_id];
-    const int kw = min(64, kmax - k0);
+    const int kw = min(BTA_GEMM_KC, kmax - k0);
     const double alpha = sg.alpha;
     pa.store(sA, sg.ta);
     pb.store(sB, sg.tb == 1 ? 0 : 1);
     __syncthreads();
     have = next_chunk(false);
     if (have) fetch();
-    mma_nt_alpha(acc, sA, sB, (kw + 3) & ~3, alpha);
+    mma_nt_alpha<BTA_GEMM_LD>(acc, sA, sB, (kw + 3) & ~3, alpha);
     __syncthreads();
   }
   // epilogue
