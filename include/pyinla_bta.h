@@ -106,8 +106,9 @@ double* bta_ws_logdet_ptr(bta_ws_t ws);
  * workgroup on the diagonal plus helper workgroups running a dependency-
  * levelled task list).  BTA_OPT_FLOW_HELPERS sets the helper count (0 =
  * automatic: half the device's resident capacity, less one) and
- * BTA_OPT_FLOW_DELTA how many tile columns far-from-diagonal updates may be
- * deferred (default 2). */
+ * BTA_OPT_FLOW_DELTA the group size: a tile away from the diagonal receives
+ * its rank-64 updates in groups of that many tile columns, back to back with
+ * its accumulator in registers (default 4; 1 = one at a time). */
 #define BTA_OPT_PANEL 3
 #define BTA_OPT_FLOW_HELPERS 4
 #define BTA_OPT_FLOW_DELTA 5

@@ -13,13 +13,15 @@
 //     C(ρ, c) -= X(ρ, k) X(c, k)^T -- from a task list built on the host.
 //
 // Every task has a level: Winv_k (chain) 3k, solves of column k 3k+1, updates
-// with column j at 3 min(j + δ, cap) + 2, where the cap keeps a tile's updates
-// before its own solve (cap = c - 1) or before the chain takes it over
-// (cap = c - 2 on the diagonal).  Each helper runs its tasks in increasing
-// level and every task depends only on lower levels, so, with all workgroups
-// resident, the lowest unfinished task can always run: no ordering can
-// deadlock.  δ > 0 defers updates of tiles far from the diagonal, which
-// evens out the per-column load of the helpers.
+// with column j at 3 min(g(j), cap) + 2, where g(j) is the last column of j's
+// group of δ columns and the cap keeps a tile's updates before its own solve
+// (cap = c - 1) or before the chain takes it over (cap = c - 2 on the
+// diagonal).  Each helper runs its tasks in increasing level and every task
+// depends only on lower levels, so, with all workgroups resident, the lowest
+// unfinished task can always run: no ordering can deadlock.  Grouping (δ > 1)
+// applies a far tile's updates from δ columns back to back with the
+// accumulator in registers (one load and one store per group); the hand-off
+// tiles on the chain's path are never grouped.
 //
 // The block update that couples block i to the next one is folded into the
 // same launch: as soon as the columns k..k+S-1 of L_i (and B_i) are final,

@@ -99,7 +99,11 @@ void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedul
       const int cap = diag ? c - 2 : c - 1;
       for (int j = 0; j < c; ++j) {
         if (diag && j == c - 1) continue;  // applied by the chain
-        const int round = hand ? j : std::min(j + delta, cap);
+        // far tiles take their updates in groups of `delta` columns at the
+        // group's last column: consecutive in the owner's list, so the
+        // accumulator stays in registers across the group
+        const int grp = std::max(delta, 1);
+        const int round = hand ? j : std::min((j / grp + 1) * grp - 1, cap);
         const int prio = hand ? 0 : 1 + (c - round);
         push(3 * round + 2, prio, FLOW_UPD, rho, c, j);
       }
@@ -474,49 +478,63 @@ int flow_capacity(int device);
 FlowPlan::~FlowPlan() { release(); }
 
 void FlowPlan::release() {
-  if (d_tasks) hipFree(d_tasks);
-  if (d_off) hipFree(d_off);
+  for (auto& sc : scheds) {
+    if (sc.d_tasks) hipFree(sc.d_tasks);
+    if (sc.d_off) hipFree(sc.d_off);
+  }
+  scheds.clear();
   if (xbuf) hipFree(xbuf);
   if (flags) hipFree(flags);
-  d_tasks = nullptr;
-  d_off = nullptr;
   xbuf = nullptr;
   flags = nullptr;
   hacc = nullptr;
   nT = -1;
 }
 
-int FlowPlan::build(int nT_, int nTa_, int helpers_, int delta_) {
+int FlowPlan::build(int nT_, int nTa_) {
   release();
   nT = nT_;
   nTa = nTa_;
-  helpers = helpers_;
-  delta = delta_;
   const int R = 2 * nT + nTa;
-  std::vector<uint32_t> all;
-  std::vector<int> off;
-  for (int v = 0; v < 2; ++v) {  // 0: block with L, 1: last block (no L)
-    FlowShape s{nT, v == 0 ? nT : 0, nTa};
-    FlowSchedule sc;
-    flow_build_schedule(s, helpers, delta, sc);
-    task_base[v] = (int)all.size();
-    off_base[v] = (int)off.size();
-    for (int h = 0; h <= helpers; ++h) off.push_back(sc.off[h] + task_base[v]);
-    all.insert(all.end(), sc.tasks.begin(), sc.tasks.end());
-  }
   hipError_t e;
-  if ((e = hipMalloc(&d_tasks, std::max<size_t>(1, all.size()) * sizeof(uint32_t))) != hipSuccess) return (int)e;
-  if ((e = hipMalloc(&d_off, off.size() * sizeof(int))) != hipSuccess) return (int)e;
-  if (!all.empty() && (e = hipMemcpy(d_tasks, all.data(), all.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
-    return (int)e;
-  if ((e = hipMemcpy(d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
-    return (int)e;
   const size_t xd = ((size_t)R * nT + 2 * (size_t)nT) * 4096;
   if ((e = hipMalloc(&xbuf, xd * sizeof(double))) != hipSuccess) return (int)e;
   hacc = xbuf + (size_t)R * nT * 4096;
   const size_t nf = (size_t)R * nT + 3 * (size_t)nT;
   if ((e = hipMalloc(&flags, nf * sizeof(unsigned long long))) != hipSuccess) return (int)e;
   if ((e = hipMemset(flags, 0, nf * sizeof(unsigned long long))) != hipSuccess) return (int)e;
+  return 0;
+}
+
+int FlowPlan::sched(int helpers, int delta, const FlowSched** out) {
+  for (const auto& sc : scheds)
+    if (sc.helpers == helpers && sc.delta == delta) {
+      *out = &sc;
+      return 0;
+    }
+  FlowSched sc;
+  sc.helpers = helpers;
+  sc.delta = delta;
+  std::vector<uint32_t> all;
+  std::vector<int> off;
+  for (int v = 0; v < 2; ++v) {  // 0: block with L, 1: last block (no L)
+    FlowShape s{nT, v == 0 ? nT : 0, nTa};
+    FlowSchedule h;
+    flow_build_schedule(s, helpers, delta, h);
+    const int base = (int)all.size();
+    sc.off_base[v] = (int)off.size();
+    for (int q = 0; q <= helpers; ++q) off.push_back(h.off[q] + base);
+    all.insert(all.end(), h.tasks.begin(), h.tasks.end());
+  }
+  hipError_t e;
+  if ((e = hipMalloc(&sc.d_tasks, std::max<size_t>(1, all.size()) * sizeof(uint32_t))) != hipSuccess) return (int)e;
+  if ((e = hipMalloc(&sc.d_off, off.size() * sizeof(int))) != hipSuccess) return (int)e;
+  if (!all.empty() && (e = hipMemcpy(sc.d_tasks, all.data(), all.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+    return (int)e;
+  if ((e = hipMemcpy(sc.d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return (int)e;
+  scheds.push_back(sc);
+  *out = &scheds.back();
   return 0;
 }
 
@@ -532,7 +550,7 @@ unsigned long long flow_spin_ticks(int device, double seconds) {
   return (unsigned long long)(seconds * 1e3 * khz);
 }
 
-FlowArgs flow_args(const FlowPlan& fp, double* D, double* L, double* B, double* T, double* winv,
+FlowArgs flow_args(const FlowPlan& fp, const FlowSched& sc, double* D, double* L, double* B, double* T, double* winv,
                    double* logpart, const unsigned long long* epoch, unsigned long long* info, int* err, int i,
                    int n, int b, int a) {
   FlowArgs p;
@@ -549,8 +567,8 @@ FlowArgs flow_args(const FlowPlan& fp, double* D, double* L, double* B, double* 
   p.logpart = logpart + (int64_t)i * nT;
   p.flags = fp.flags;
   const int v = (i < n - 1) ? 0 : 1;
-  p.tasks = fp.d_tasks;
-  p.off = fp.d_off + fp.off_base[v];
+  p.tasks = sc.d_tasks;
+  p.off = sc.d_off + sc.off_base[v];
   p.epoch = epoch;
   p.info = info;
   p.err = err;
@@ -562,14 +580,14 @@ FlowArgs flow_args(const FlowPlan& fp, double* D, double* L, double* B, double* 
   p.nTL = (i < n - 1) ? nT : 0;
   p.nTa = (a > 0) ? fp.nTa : 0;
   p.blk = i + 1;
-  p.helpers = fp.helpers;
+  p.helpers = sc.helpers;
   return p;
 }
 
-void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, double* T, int i,
-                       int n, int b, int a, hipStream_t st) {
-  const FlowArgs p = flow_args(fp, D, L, B, T, ws.winv, ws.logpart, ws.epoch, ws.info, ws.err, i, n, b, a);
-  hipLaunchKernelGGL(k_panel_flow, dim3(1 + fp.helpers), dim3(BTA_THREADS), 0, st, p);
+void flow_launch_block(const Workspace& ws, const FlowPlan& fp, const FlowSched& sc, double* D, double* L,
+                       double* B, double* T, int i, int n, int b, int a, hipStream_t st) {
+  const FlowArgs p = flow_args(fp, sc, D, L, B, T, ws.winv, ws.logpart, ws.epoch, ws.info, ws.err, i, n, b, a);
+  hipLaunchKernelGGL(k_panel_flow, dim3(1 + sc.helpers), dim3(BTA_THREADS), 0, st, p);
 }
 
 // Batched form: entry e of a launch owns workgroups [begin[e], begin[e+1]):

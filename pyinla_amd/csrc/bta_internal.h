@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <deque>
 #include <memory>
 #include <utility>
 #include <vector>
@@ -32,29 +33,38 @@ struct TimingLog {
   void end(int cls, hipStream_t st);
 };
 
-// Schedules and exchange buffers of the persistent panel launch (bta_flow.hip)
-// for one (nT, nTa, helpers, delta); variant 0 is a block with L, variant 1
-// the last block.
-struct FlowPlan {
+// Device copy of the helper task lists of the persistent panel launch for one
+// (nT, nTa, helpers, delta); variant 0 is a block with L, variant 1 the last
+// block.
+struct FlowSched {
   uint32_t* d_tasks = nullptr;
   int* d_off = nullptr;
+  int off_base[2] = {0, 0};
+  int helpers = 0, delta = 0;
+};
+
+// Exchange buffers of the persistent panel launch for one panel shape, plus the
+// schedules built for it (one per helper count / group size in use).
+struct FlowPlan {
   double* xbuf = nullptr;   // (2 nT + nTa) * nT tiles, then 2 nT hand-off tiles
   double* hacc = nullptr;
   unsigned long long* flags = nullptr;
-  int task_base[2] = {0, 0}, off_base[2] = {0, 0};
-  int nT = -1, nTa = 0, helpers = 0, delta = 0;
+  int nT = -1, nTa = 0;
   unsigned long long spin_ticks = 0;
+  std::deque<FlowSched> scheds;  // deque: pointers to entries stay valid
   FlowPlan() = default;
   FlowPlan(const FlowPlan&) = delete;
   FlowPlan& operator=(const FlowPlan&) = delete;
   ~FlowPlan();
   void release();
-  int build(int nT, int nTa, int helpers, int delta);
+  int build(int nT, int nTa);
+  // schedule for (helpers, delta), built on first request (host; not during capture)
+  int sched(int helpers, int delta, const FlowSched** out);
 };
 
 int flow_helpers_default(int device);
 int flow_capacity(int device);  // co-resident panel workgroups on the device
-FlowArgs flow_args(const FlowPlan& fp, double* D, double* L, double* B, double* T, double* winv,
+FlowArgs flow_args(const FlowPlan& fp, const FlowSched& sc, double* D, double* L, double* B, double* T, double* winv,
                    double* logpart, const unsigned long long* epoch, unsigned long long* info, int* err, int i,
                    int n, int b, int a);
 void flow_launch_batched(const FlowArgs* d_args, const int* d_begin, int nent, int total, hipStream_t st);
@@ -100,8 +110,9 @@ struct Workspace {
   // 2 = persistent dataflow launch per block (bta_flow.hip).
   int panel_mode = 0;
   int flow_helpers = 0;  // 0 = automatic
-  int flow_delta = 2;
+  int flow_delta = 4;    // update group of far tiles (columns)
   std::unique_ptr<FlowPlan> flow;
+  const FlowSched* flow_sched = nullptr;
 };
 
 inline int choose_tile(int pref, int m, int n) { return pref ? pref : pick_tile(m, n); }
@@ -165,8 +176,8 @@ struct BatchPlan {
 int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a);
 // One persistent launch: panel of block i plus its updates of D_{i+1},
 // B_{i+1} and T.
-void flow_launch_block(const Workspace& ws, const FlowPlan& fp, double* D, double* L, double* B, double* T, int i,
-                       int n, int b, int a, hipStream_t st);
+void flow_launch_block(const Workspace& ws, const FlowPlan& fp, const FlowSched& sc, double* D, double* L,
+                       double* B, double* T, int i, int n, int b, int a, hipStream_t st);
 int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
                   hipStream_t st);
 // Host-side preparation (allocations, job tables); never called during capture.

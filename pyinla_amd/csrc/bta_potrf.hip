@@ -433,17 +433,17 @@ static int flow_prepare(Workspace& ws, int b, int a) {
   int helpers = ws.flow_helpers > 0 ? ws.flow_helpers : flow_helpers_default(ws.device);
   if (helpers <= 0) return (int)hipErrorInvalidValue;
   helpers = std::min(helpers, 1023);
-  if (ws.flow && ws.flow->nT == nT && ws.flow->nTa == nTa && ws.flow->helpers == helpers &&
-      ws.flow->delta == ws.flow_delta)
-    return 0;
-  ws.flow.reset(new FlowPlan());
-  const int rc = ws.flow->build(nT, nTa, helpers, ws.flow_delta);
-  if (rc) {
-    ws.flow.reset();
-    return rc;
+  if (!(ws.flow && ws.flow->nT == nT && ws.flow->nTa == nTa)) {
+    ws.flow.reset(new FlowPlan());
+    ws.flow_sched = nullptr;
+    const int rc = ws.flow->build(nT, nTa);
+    if (rc) {
+      ws.flow.reset();
+      return rc;
+    }
+    ws.flow->spin_ticks = flow_spin_ticks(ws.device, 20.0);
   }
-  ws.flow->spin_ticks = flow_spin_ticks(ws.device, 20.0);
-  return 0;
+  return ws.flow->sched(helpers, ws.flow_delta, &ws.flow_sched);
 }
 
 int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a) {
@@ -499,7 +499,7 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
   for (int q = 0; q < 4 && use_plan; ++q) use_plan = (ws.update_key[q] == key[q]);
   const int nT = (b + 63) / 64;
   const int nTa = (a + 63) / 64;
-  const bool flow = use_flow(ws, b) && ws.flow && ws.flow->nT == nT && ws.flow->nTa == nTa;
+  const bool flow = use_flow(ws, b) && ws.flow && ws.flow->nT == nT && ws.flow->nTa == nTa && ws.flow_sched;
   if (use_flow(ws, b) && !flow) return (int)hipErrorNotReady;  // potrf_prepare not run for this shape
   hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, st, ws.epoch, ws.info, ws.err);
   for (int i = 0; i < n; ++i) {
@@ -522,7 +522,7 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     p.nTa = p.B ? nTa : 0;
     if (flow) {  // the launch also applies the block update
       TimedScope ts(ws, KC_PANEL, st);
-      flow_launch_block(ws, *ws.flow, D, L, B, T, i, n, b, a, st);
+      flow_launch_block(ws, *ws.flow, *ws.flow_sched, D, L, B, T, i, n, b, a, st);
       continue;
     } else {
       for (int k = 0; k < nT; ++k) {
@@ -572,10 +572,11 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int device
   for (const auto& d : ps) all_wide = all_wide && d.b > 64;
   flow = (panel_mode == 2) || (panel_mode == 0 && all_wide);
   if (flow) {
+    // helpers per problem: the device's resident capacity split evenly over
+    // the problems still active at each block step
     const int cap = flow_capacity(device);
     const int np = (int)ps.size();
-    const int helpers = std::min(1023, cap / np - 1);
-    if (helpers < 1) {
+    if (cap / np - 1 < 1) {
       flow = false;  // too many problems to give each a resident chain + helper
     } else {
       std::vector<FlowArgs> args;
@@ -585,11 +586,14 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int device
       fplans.clear();
       for (const auto& d : ps) {
         fplans.emplace_back(new FlowPlan());
-        const int rc = fplans.back()->build(d.nT, d.a > 0 ? d.nTa : 0, helpers, 2);
+        const int rc = fplans.back()->build(d.nT, d.a > 0 ? d.nTa : 0);
         if (rc) return rc;
         fplans.back()->spin_ticks = flow_spin_ticks(device, 20.0);
       }
       for (int i = 0; i < nmax; ++i) {
+        int nact = 0;
+        for (const auto& d : ps) nact += (i < d.n) ? 1 : 0;
+        const int helpers = std::min(1023, cap / nact - 1);
         FLaunch L;
         L.off_args = (int)args.size();
         L.off_begin = (int)begin.size();
@@ -597,8 +601,11 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int device
         for (int q = 0; q < np; ++q) {
           const ProbDesc& d = ps[q];
           if (i >= d.n) continue;
-          args.push_back(flow_args(*fplans[q], d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch, d.info, d.err, i,
-                                   d.n, d.b, d.a));
+          const FlowSched* sc = nullptr;
+          const int rc = fplans[q]->sched(helpers, 4, &sc);
+          if (rc) return rc;
+          args.push_back(flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch, d.info, d.err,
+                                   i, d.n, d.b, d.a));
           begin.push_back(total);
           total += 1 + helpers;
         }
