@@ -284,7 +284,7 @@ int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B
   bta_batch* bt = new bta_batch();
   bt->device = ws[0]->w.device;
   bt->ws.assign(ws, ws + nprob);
-  const int rc = bt->plan.build(ps, ws[0]->w.panel_mode, ws[0]->w.device);
+  const int rc = bt->plan.build(ps, ws[0]->w.panel_mode, ws[0]->w.batch_lanes, ws[0]->w.device);
   if (rc) {
     delete bt;
     return rc;
@@ -377,12 +377,14 @@ int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   if (option == BTA_OPT_PANEL && (value < 0 || value > 2)) return BTA_E_ARG;
   if (option == BTA_OPT_FLOW_HELPERS && (value < 0 || value > 1023)) return BTA_E_ARG;
   if (option == BTA_OPT_FLOW_DELTA && (value < 0 || value > 64)) return BTA_E_ARG;
+  if (option == BTA_OPT_BATCH_LANES && (value < 0 || value > 512)) return BTA_E_ARG;
   std::lock_guard<std::mutex> g(ws->mu);
   if (option == BTA_OPT_TILE_UPDATE) ws->w.tile_update = value;
   else if (option == BTA_OPT_TILE_GROUPED) ws->w.tile_grouped = value;
   else if (option == BTA_OPT_PANEL) ws->w.panel_mode = value;
   else if (option == BTA_OPT_FLOW_HELPERS) ws->w.flow_helpers = value;
   else if (option == BTA_OPT_FLOW_DELTA) ws->w.flow_delta = value;
+  else if (option == BTA_OPT_BATCH_LANES) ws->w.batch_lanes = value;
   else return BTA_E_ARG;
   // recorded graphs embed the old launch shapes
   for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);

@@ -64,6 +64,7 @@ void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedul
   };
   std::vector<Item> v;
   const int nT = s.nT, R = s.rows();
+  const int hn = flow_near_pool(s, helpers);  // near tiles are never grouped
   auto push = [&](int level, int prio, int op, int rho, int c, int k) {
     const unsigned long long key = ((unsigned long long)level << 42) | ((unsigned long long)prio << 31) |
                                    ((unsigned long long)c << 21) | ((unsigned long long)rho << 10) |
@@ -102,7 +103,7 @@ void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedul
         // far tiles take their updates in groups of `delta` columns at the
         // group's last column: consecutive in the owner's list, so the
         // accumulator stays in registers across the group
-        const int grp = std::max(delta, 1);
+        const int grp = (hn > 0 && flow_is_near(nT, rho, c)) ? 1 : std::max(delta, 1);
         const int round = hand ? j : std::min((j / grp + 1) * grp - 1, cap);
         const int prio = hand ? 0 : 1 + (c - round);
         push(3 * round + 2, prio, FLOW_UPD, rho, c, j);

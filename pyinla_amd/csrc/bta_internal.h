@@ -111,6 +111,7 @@ struct Workspace {
   int panel_mode = 0;
   int flow_helpers = 0;  // 0 = automatic
   int flow_delta = 4;    // update group of far tiles (columns)
+  int batch_lanes = 0;   // problems per wave of a batched factorization (0 = 6)
   std::unique_ptr<FlowPlan> flow;
   const FlowSched* flow_sched = nullptr;
 };
@@ -167,7 +168,8 @@ struct BatchPlan {
   BatchPlan(const BatchPlan&) = delete;
   BatchPlan& operator=(const BatchPlan&) = delete;
   ~BatchPlan();
-  int build(const std::vector<ProbDesc>& ps, int panel_mode, int device);
+  // lanes: problems factored concurrently in the persistent form (0 = 6)
+  int build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int device);
   int enqueue(hipStream_t st) const;
 };
 

@@ -563,7 +563,7 @@ BatchPlan::~BatchPlan() {
   if (d_fbegin) hipFree(d_fbegin);
 }
 
-int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int device) {
+int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int device) {
   probs = ps;
   // Persistent dataflow form when every problem has more than one tile column
   // (or when asked for): each problem gets a chain and an equal share of the
@@ -576,7 +576,8 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int device
     // the problems still active at each block step
     const int cap = flow_capacity(device);
     const int np = (int)ps.size();
-    if (cap / np - 1 < 1) {
+    const int nl = lanes > 0 ? std::min(lanes, np) : std::min(np, 6);
+    if (cap / nl - 1 < 1) {
       flow = false;  // too many problems to give each a resident chain + helper
     } else {
       std::vector<FlowArgs> args;
@@ -590,30 +591,43 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int device
         if (rc) return rc;
         fplans.back()->spin_ticks = flow_spin_ticks(device, 20.0);
       }
-      for (int i = 0; i < nmax; ++i) {
-        int nact = 0;
-        for (const auto& d : ps) nact += (i < d.n) ? 1 : 0;
-        const int helpers = std::min(1023, cap / nact - 1);
-        FLaunch L;
-        L.off_args = (int)args.size();
-        L.off_begin = (int)begin.size();
-        int total = 0;
-        for (int q = 0; q < np; ++q) {
-          const ProbDesc& d = ps[q];
-          if (i >= d.n) continue;
-          const FlowSched* sc = nullptr;
-          const int rc = fplans[q]->sched(helpers, 4, &sc);
-          if (rc) return rc;
-          args.push_back(flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch, d.info, d.err,
-                                   i, d.n, d.b, d.a));
+      // Problems run in waves of at most `lanes` (longest first), so that each
+      // chain gets enough helpers; within a wave every launch covers the
+      // wave's problems that still have block step i.
+      std::vector<int> order(np);
+      for (int q = 0; q < np; ++q) order[q] = q;
+      std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ps[x].n > ps[y].n; });
+      for (int w0 = 0; w0 < np; w0 += nl) {
+        const int w1 = std::min(np, w0 + nl);
+        int wmax = 0;
+        for (int t = w0; t < w1; ++t) wmax = std::max(wmax, ps[order[t]].n);
+        for (int i = 0; i < wmax; ++i) {
+          int nact = 0;
+          for (int t = w0; t < w1; ++t) nact += (i < ps[order[t]].n) ? 1 : 0;
+          const int helpers = std::min(1023, cap / nact - 1);
+          FLaunch L;
+          L.off_args = (int)args.size();
+          L.off_begin = (int)begin.size();
+          int total = 0;
+          for (int t = w0; t < w1; ++t) {
+            const int q = order[t];
+            const ProbDesc& d = ps[q];
+            if (i >= d.n) continue;
+            const FlowSched* sc = nullptr;
+            const int rc = fplans[q]->sched(helpers, 4, &sc);
+            if (rc) return rc;
+            args.push_back(flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch, d.info,
+                                     d.err, i, d.n, d.b, d.a));
+            begin.push_back(total);
+            total += 1 + helpers;
+          }
           begin.push_back(total);
-          total += 1 + helpers;
+          L.nent = (int)args.size() - L.off_args;
+          L.total = total;
+          flaunches.push_back(L);
         }
-        begin.push_back(total);
-        L.nent = (int)args.size() - L.off_args;
-        L.total = total;
-        flaunches.push_back(L);
       }
+      (void)nmax;
       hipError_t e;
       if ((e = hipMalloc(&d_fargs, args.size() * sizeof(FlowArgs))) != hipSuccess) return (int)e;
       if ((e = hipMemcpy(d_fargs, args.data(), args.size() * sizeof(FlowArgs), hipMemcpyHostToDevice)) !=
