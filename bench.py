@@ -76,6 +76,8 @@ def parse():
                     help="adjoint: the headline line; the others measure the batched FD stencil, the "
                          "mixed-size batch and coregional L-BFGS (their own JSON lines)")
     ap.add_argument("--fd-wave", type=int, default=0, help="stencil points per batched FD wave (0 = all)")
+    ap.add_argument("--batch-lanes", type=int, nargs="*", default=[],
+                    help="mixed batch: also time these lane counts (problems run side by side)")
     ap.add_argument("--lbfgs-iters", type=int, default=20)
     ap.add_argument("--lbfgs-fd-iters", type=int, default=1)
     return ap.parse_args()
@@ -201,11 +203,25 @@ def bench_mixed_batch(args, dev):
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
     ld_s = [s.status()[1] for s in solvers]
+    lanes_s = {}
+    for lanes in args.batch_lanes:
+        solvers[0].set_batch_lanes(lanes)
+        bl = BatchedFactorization(solvers, ["bta"] * len(solvers))
+        load_all()
+        bl.factorize()
+        load_all()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bl.factorize()
+        torch.cuda.synchronize()
+        lanes_s[str(lanes)] = time.perf_counter() - t0
+        del bl
+    solvers[0].set_batch_lanes(0)
     return {"metric": "mixed batch of 24 BTA factorizations", "unit": "TFLOP/s",
             "config": {"workload": "24 known-factor BTA matrices, nt in {50,100,200}, b in {256,512,1024,2048}, a=6",
                        "shapes": shapes},
             "batched_s": t_bat, "sequential_s": t_seq, "batched_tflops": total / t_bat / 1e12,
-            "sequential_tflops": total / t_seq / 1e12,
+            "sequential_tflops": total / t_seq / 1e12, "batched_s_by_lanes": lanes_s,
             "max_logdet_rel_diff": max(abs(x - y) / abs(y) for x, y in zip(ld_b, ld_s)),
             "max_logdet_rel_err_vs_known_factor": max(abs(x - y) / abs(y) for x, y in zip(ld_b, ld_exact)),
             "construction": "config-0 known factor, off-diagonal draws scaled by sqrt(92/b)"}

@@ -106,9 +106,10 @@ double* bta_ws_logdet_ptr(bta_ws_t ws);
  * workgroup on the diagonal plus helper workgroups running a dependency-
  * levelled task list).  BTA_OPT_FLOW_HELPERS sets the helper count (0 =
  * automatic: half the device's resident capacity, less one) and
- * BTA_OPT_FLOW_DELTA the group size: a tile away from the diagonal receives
- * its rank-64 updates in groups of that many tile columns, back to back with
- * its accumulator in registers (default 4; 1 = one at a time). */
+ * BTA_OPT_FLOW_DELTA the order of the helpers' task lists: 0 (default) = the
+ * order of a host-side simulation of the launch, critical path first; d >= 1
+ * = dependency-level order, a tile away from the diagonal receiving its
+ * rank-64 updates in groups of d tile columns. */
 #define BTA_OPT_PANEL 3
 #define BTA_OPT_FLOW_HELPERS 4
 #define BTA_OPT_FLOW_DELTA 5

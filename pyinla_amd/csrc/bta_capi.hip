@@ -284,7 +284,7 @@ int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B
   bta_batch* bt = new bta_batch();
   bt->device = ws[0]->w.device;
   bt->ws.assign(ws, ws + nprob);
-  const int rc = bt->plan.build(ps, ws[0]->w.panel_mode, ws[0]->w.batch_lanes, ws[0]->w.device);
+  const int rc = bt->plan.build(ps, ws[0]->w.panel_mode, ws[0]->w.batch_lanes, ws[0]->w.flow_delta, ws[0]->w.device);
   if (rc) {
     delete bt;
     return rc;
@@ -361,7 +361,7 @@ int bta_flow_schedule(int nT, int nTL, int nTa, int helpers, int delta, uint32_t
       2 * nT + nTa > 1023)
     return BTA_E_ARG;
   bta::FlowSchedule sc;
-  bta::flow_build_schedule(bta::FlowShape{nT, nTL, nTa}, helpers, delta, sc);
+  if (!bta::flow_build_schedule(bta::FlowShape{nT, nTL, nTa}, helpers, delta, sc)) return BTA_E_ARG;
   if (ntasks) *ntasks = (int64_t)sc.tasks.size();
   if (!tasks && !off) return 0;
   if (!tasks || !off || cap < (int64_t)sc.tasks.size()) return BTA_E_ARG;

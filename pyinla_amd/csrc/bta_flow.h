@@ -113,6 +113,9 @@ struct FlowArgs {
 };
 
 int flow_owner(const FlowShape& s, int rho, int c, int helpers);
-void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedule& out);
+// delta >= 1: level order, far tiles' updates grouped by delta columns;
+// delta == 0: list order from a simulated launch.  False on an internal
+// inconsistency (a dependency cycle).
+bool flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedule& out);
 
 }  // namespace bta

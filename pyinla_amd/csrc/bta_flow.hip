@@ -9,6 +9,7 @@
 // L_{i+1,i} in L_i, L_{a,i} in B_i, inverse of every diagonal tile in winv and
 // per-tile log-pivot sums in logpart.
 #include <algorithm>
+#include <queue>
 
 #include "bta_internal.h"
 #include "bta_flow.h"
@@ -56,7 +57,9 @@ static int flow_next_owner(const FlowShape& s, int rho, int c, int helpers) {
   return hn + (int)(t % (helpers - hn));
 }
 
-void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedule& out) {
+// Level order (delta >= 1): each helper runs its tasks by dependency level;
+// far tiles take their updates in groups of `delta` columns.
+static void flow_build_levels(const FlowShape& s, int helpers, int delta, FlowSchedule& out) {
   struct Item {
     unsigned long long key;
     uint32_t word;
@@ -122,6 +125,243 @@ void flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedul
     out.off[v[q].h + 1] += 1;
   }
   for (int h = 0; h < helpers; ++h) out.off[h + 1] += out.off[h];
+}
+
+// List order (delta == 0).  The launch is simulated on the host with the task
+// costs below (core clock cycles, from s_memtime stamps on MI355X): whenever a
+// helper is idle it starts the ready task of highest upward rank (its cost
+// plus the longest path of costs and flag latencies from it to the end of the
+// launch), so the solves and updates the chain is about to wait for come
+// before bulk updates of tiles needed much later and the next-block updates
+// fill the gaps.  Each helper's list is the order of its tasks in that
+// simulation.  The simulated start times order all tasks, the chain's
+// included, consistently with every dependency and with every list, so with
+// all workgroups resident the earliest unfinished task in that order can
+// always run, whatever the real durations: no interleaving can deadlock.
+#ifndef FLOW_COST_CHOL
+#define FLOW_COST_CHOL 63000  // chain: factor + inverse of a diagonal tile, stores
+#endif
+#ifndef FLOW_COST_CTRSM
+#define FLOW_COST_CTRSM 8000  // chain: solve of the sub-diagonal tile, stores
+#endif
+#ifndef FLOW_COST_CUPD
+#define FLOW_COST_CUPD 8000  // chain: update of the next diagonal tile
+#endif
+#ifndef FLOW_COST_UPD
+#define FLOW_COST_UPD 12600  // rank-64 update, accumulator loaded and stored
+#endif
+#ifndef FLOW_COST_CARRY
+#define FLOW_COST_CARRY 3000  // saved when the previous task left the accumulator in registers
+#endif
+#ifndef FLOW_COST_TRSM
+#define FLOW_COST_TRSM 12500
+#endif
+#ifndef FLOW_COST_NEXTCOL
+#define FLOW_COST_NEXTCOL 10800  // one column of a next-block task
+#endif
+#ifndef FLOW_COST_ZERO
+#define FLOW_COST_ZERO 500
+#endif
+#ifndef FLOW_COST_LAT
+#define FLOW_COST_LAT 1500  // flag store to flag seen by another workgroup
+#endif
+
+static bool flow_build_list(const FlowShape& s, int helpers, FlowSchedule& out) {
+  const int nT = s.nT, R = s.rows();
+  const int chain = helpers;  // agent id of the chain
+  std::vector<uint32_t> word;
+  std::vector<int> agent, cost, tile, op;
+  auto add = [&](int a, int c, int o, uint32_t w, int t) {
+    agent.push_back(a);
+    cost.push_back(c);
+    op.push_back(o);
+    word.push_back(w);
+    tile.push_back(t);
+    return (int)agent.size() - 1;
+  };
+  std::vector<std::vector<int>> succ;
+  std::vector<int> npred;
+  auto edge = [&](int from, int to) {
+    if (from < 0 || to < 0) return;
+    if ((int)succ.size() < (int)agent.size()) succ.resize(agent.size());
+    succ[from].push_back(to);
+  };
+  // chain: CF(k) factor, CT(k) solve of (k+1, k), CU(k) update of (k+1, k+1)
+  std::vector<int> cf(nT, -1), ct(nT, -1), cu(nT, -1);
+  for (int k = 0; k < nT; ++k) {
+    cf[k] = add(chain, FLOW_COST_CHOL, -1, 0, -1);
+    if (k + 1 < nT) {
+      ct[k] = add(chain, FLOW_COST_CTRSM, -1, 0, -1);
+      cu[k] = add(chain, FLOW_COST_CUPD, -1, 0, -1);
+    }
+  }
+  // producer of X(rho, c); last helper update of each tile
+  std::vector<int> xprod((size_t)R * nT, -1), last_upd((size_t)R * nT, -1);
+  std::vector<std::pair<int, int>> upd_in;  // (task, j) for the X edges
+  for (int k = 0; k + 1 < nT; ++k) xprod[(size_t)(k + 1) * nT + k] = ct[k];
+  for (int rho = 0; rho < R; ++rho)
+    for (int c = 0; c < nT; ++c) {
+      const int owner = flow_owner(s, rho, c, helpers);
+      const int tid = rho * nT + c;
+      if (rho < nT && rho < c) {
+        add(owner, FLOW_COST_ZERO, FLOW_ZERO, flow_encode(FLOW_ZERO, rho, c, 0), -1);
+        continue;
+      }
+      const bool diag = (rho == c);
+      const bool hand = flow_is_handoff(nT, rho, c);
+      int prev = -1;
+      for (int j = 0; j < c; ++j) {
+        if (diag && j == c - 1) continue;  // applied by the chain
+        const int t = add(owner, FLOW_COST_UPD, FLOW_UPD, flow_encode(FLOW_UPD, rho, c, j), hand ? -1 : tid);
+        upd_in.push_back({t, j});
+        edge(prev, t);  // one accumulator: updates of a tile in column order
+        prev = t;
+      }
+      last_upd[tid] = prev;
+      if (!diag && !(rho < nT && rho == c + 1)) {
+        const int t = add(owner, FLOW_COST_TRSM, FLOW_TRSM, flow_encode(FLOW_TRSM, rho, c, c), tid);
+        edge(prev, t);
+        edge(cf[c], t);
+        xprod[tid] = t;
+      }
+    }
+  // next-block updates: one task per target and span of FLOW_SPAN columns
+  auto push_next = [&](int rho, int c) {
+    const int ra = rho < nT ? nT + rho : nT + s.nTL;
+    const int rb = rho < nT ? nT + c : (rho == nT ? nT + c : nT + s.nTL);
+    const int owner = flow_next_owner(s, rho, c, helpers);
+    int prev = -1;
+    for (int k = 0; k < nT; k += FLOW_SPAN) {
+      const int kend = std::min(k + FLOW_SPAN, nT);
+      const int t = add(owner, FLOW_COST_NEXTCOL * (kend - k), FLOW_NEXT, flow_encode(FLOW_NEXT, rho, c, k), -1);
+      edge(prev, t);
+      prev = t;
+      for (int kk = k; kk < kend; ++kk) {
+        edge(xprod[(size_t)ra * nT + kk], t);
+        if (rb != ra) edge(xprod[(size_t)rb * nT + kk], t);
+      }
+    }
+  };
+  succ.resize(agent.size());
+  for (auto& u : upd_in) {
+    int o, rho, c, j;
+    flow_decode(word[u.first], o, rho, c, j);
+    edge(xprod[(size_t)rho * nT + j], u.first);
+    if (rho != c) edge(xprod[(size_t)c * nT + j], u.first);
+  }
+  if (s.nTL > 0) {
+    for (int r = 0; r < nT; ++r)
+      for (int c = 0; c <= r; ++c) push_next(r, c);
+    if (s.nTa > 0)
+      for (int c = 0; c < nT; ++c) push_next(nT, c);
+  }
+  if (s.nTa > 0) push_next(nT + 1, 0);
+  succ.resize(agent.size());
+  // chain order and its waits on the hand-off tiles
+  for (int k = 0; k < nT; ++k) {
+    if (k + 1 < nT) {
+      edge(cf[k], ct[k]);
+      edge(ct[k], cu[k]);
+      edge(cu[k], cf[k + 1]);
+      if (k >= 1) {  // the chain fetches both hand-off tiles before the solve
+        edge(last_upd[(size_t)(k + 1) * nT + k], ct[k]);
+        edge(last_upd[(size_t)(k + 1) * nT + k + 1], ct[k]);
+      }
+    }
+  }
+  const int N = (int)agent.size();
+  npred.assign(N, 0);
+  for (int t = 0; t < N; ++t)
+    for (int v : succ[t]) ++npred[v];
+  // topological order, then upward ranks
+  std::vector<int> topo;
+  topo.reserve(N);
+  {
+    std::vector<int> np = npred;
+    for (int t = 0; t < N; ++t)
+      if (np[t] == 0) topo.push_back(t);
+    for (size_t q = 0; q < topo.size(); ++q)
+      for (int v : succ[topo[q]])
+        if (--np[v] == 0) topo.push_back(v);
+  }
+  if ((int)topo.size() != N) return false;  // dependency cycle
+  std::vector<double> rank(N, 0.0);
+  for (int q = N - 1; q >= 0; --q) {
+    const int t = topo[q];
+    double m = 0.0;
+    for (int v : succ[t]) m = std::max(m, rank[v] + (agent[v] != agent[t] ? FLOW_COST_LAT : 0));
+    rank[t] = cost[t] + m;
+  }
+  // list-scheduling simulation
+  const int A = helpers + 1;
+  using RQ = std::priority_queue<std::pair<double, int>>;                       // (rank, -task)
+  using FQ = std::priority_queue<std::pair<double, int>, std::vector<std::pair<double, int>>,
+                                 std::greater<std::pair<double, int>>>;         // (ready time, task)
+  std::vector<RQ> ready(A);
+  std::vector<FQ> future(A);
+  std::vector<double> rtime(N, 0.0), busy(A, -1.0);
+  std::vector<int> carried(A, -1), left = npred;
+  std::vector<std::vector<int>> order(A);
+  struct Ev {
+    double t;
+    int kind, x;  // kind 0: task x finishes, 1: agent x looks for work
+    bool operator>(const Ev& o) const { return t != o.t ? t > o.t : kind > o.kind; }
+  };
+  std::priority_queue<Ev, std::vector<Ev>, std::greater<Ev>> ev;
+  for (int t = 0; t < N; ++t)
+    if (left[t] == 0) future[agent[t]].push({0.0, t});
+  for (int a = 0; a < A; ++a) ev.push({0.0, 1, a});
+  int done = 0;
+  while (!ev.empty()) {
+    const Ev e = ev.top();
+    ev.pop();
+    if (e.kind == 0) {
+      const int t = e.x, a = agent[t];
+      ++done;
+      for (int v : succ[t]) {
+        rtime[v] = std::max(rtime[v], e.t + (agent[v] != a ? FLOW_COST_LAT : 0));
+        if (--left[v] == 0) {
+          future[agent[v]].push({rtime[v], v});
+          ev.push({rtime[v], 1, agent[v]});
+        }
+      }
+      ev.push({e.t, 1, a});
+      continue;
+    }
+    const int a = e.x;
+    if (busy[a] > e.t) continue;  // still running a task; its finish re-posts
+    while (!future[a].empty() && future[a].top().first <= e.t) {
+      const int t = future[a].top().second;
+      future[a].pop();
+      ready[a].push({rank[t], -t});
+    }
+    if (ready[a].empty()) continue;  // woken again when a task becomes ready
+    const int t = -ready[a].top().second;
+    ready[a].pop();
+    int c = cost[t];
+    if (tile[t] >= 0 && tile[t] == carried[a]) c -= FLOW_COST_CARRY;
+    carried[a] = (op[t] == FLOW_UPD && tile[t] >= 0) ? tile[t] : -1;
+    busy[a] = e.t + c;
+    order[a].push_back(t);
+    ev.push({busy[a], 0, t});
+  }
+  if (done != N) return false;
+  out.helpers = helpers;
+  out.tasks.clear();
+  out.off.assign(helpers + 1, 0);
+  for (int h = 0; h < helpers; ++h) {
+    for (int t : order[h]) out.tasks.push_back(word[t]);
+    out.off[h + 1] = (int)out.tasks.size();
+  }
+  return true;
+}
+
+bool flow_build_schedule(const FlowShape& s, int helpers, int delta, FlowSchedule& out) {
+  if (delta >= 1) {
+    flow_build_levels(s, helpers, delta, out);
+    return true;
+  }
+  return flow_build_list(s, helpers, out);
 }
 
 // ------------------------------------------------------------------ device
@@ -521,7 +761,7 @@ int FlowPlan::sched(int helpers, int delta, const FlowSched** out) {
   for (int v = 0; v < 2; ++v) {  // 0: block with L, 1: last block (no L)
     FlowShape s{nT, v == 0 ? nT : 0, nTa};
     FlowSchedule h;
-    flow_build_schedule(s, helpers, delta, h);
+    if (!flow_build_schedule(s, helpers, delta, h)) return -1;
     const int base = (int)all.size();
     sc.off_base[v] = (int)off.size();
     for (int q = 0; q <= helpers; ++q) off.push_back(h.off[q] + base);

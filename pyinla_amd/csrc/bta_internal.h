@@ -110,7 +110,7 @@ struct Workspace {
   // 2 = persistent dataflow launch per block (bta_flow.hip).
   int panel_mode = 0;
   int flow_helpers = 0;  // 0 = automatic
-  int flow_delta = 4;    // update group of far tiles (columns)
+  int flow_delta = 0;    // 0: list-scheduled task order; >= 1: level order, far-tile update groups
   int batch_lanes = 0;   // problems per wave of a batched factorization (0 = 6)
   std::unique_ptr<FlowPlan> flow;
   const FlowSched* flow_sched = nullptr;
@@ -169,7 +169,7 @@ struct BatchPlan {
   BatchPlan& operator=(const BatchPlan&) = delete;
   ~BatchPlan();
   // lanes: problems factored concurrently in the persistent form (0 = 6)
-  int build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int device);
+  int build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int delta, int device);
   int enqueue(hipStream_t st) const;
 };
 

@@ -563,7 +563,7 @@ BatchPlan::~BatchPlan() {
   if (d_fbegin) hipFree(d_fbegin);
 }
 
-int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int device) {
+int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int delta, int device) {
   probs = ps;
   // Persistent dataflow form when every problem has more than one tile column
   // (or when asked for): each problem gets a chain and an equal share of the
@@ -614,7 +614,7 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
             const ProbDesc& d = ps[q];
             if (i >= d.n) continue;
             const FlowSched* sc = nullptr;
-            const int rc = fplans[q]->sched(helpers, 4, &sc);
+            const int rc = fplans[q]->sched(helpers, delta, &sc);
             if (rc) return rc;
             args.push_back(flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch, d.info,
                                      d.err, i, d.n, d.b, d.a));

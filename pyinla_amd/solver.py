@@ -114,8 +114,10 @@ class SerinvSolver:
     def set_panel(self, mode: str = "auto", helpers: int = 0, delta: int | None = None) -> None:
         """Panel factorization variant of ``factorize`` (no reference
         counterpart): "auto", "columns" (one launch per 64-wide tile column)
-        or "flow" (one persistent dataflow launch per block); ``helpers`` and
-        ``delta`` tune the flow launch (0 = automatic helper count)."""
+        or "flow" (one persistent dataflow launch per block); ``helpers`` sets
+        the flow launch's helper workgroups (0 = automatic) and ``delta`` the
+        order of their task lists (0 = simulated critical-path order, d >= 1 =
+        dependency levels with far-tile updates grouped by d columns)."""
         modes = {"auto": 0, "columns": 1, "flow": 2}
         if mode not in modes:
             raise ValueError(f"unknown panel mode {mode!r}")
@@ -123,6 +125,12 @@ class SerinvSolver:
         _lib.check(self.lib.bta_ws_set_option(self._ws, 4, int(helpers)), "set_option")
         if delta is not None:
             _lib.check(self.lib.bta_ws_set_option(self._ws, 5, int(delta)), "set_option")
+
+    def set_batch_lanes(self, lanes: int = 0) -> None:
+        """Problems factored side by side when this solver is the first member
+        of a ``BatchedFactorization`` built afterwards (no reference
+        counterpart; 0 = automatic)."""
+        _lib.check(self.lib.bta_ws_set_option(self._ws, 6, int(lanes)), "set_option")
 
     def status(self):
         """(info, logdet, err) of the last factorization (synchronises)."""
