@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--fd-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-nt", type=int, default=4, help="time steps of the bounded CPU sample")
-    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay solver operations from hipGraphs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (a known-size kernel to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters against)")
@@ -288,7 +288,7 @@ def main():
                       hx=1.0 / (args.nx - 1), hy=1.0 / (args.ny - 1))
     n, b, a = prob.nt, prob.ns, prob.nb
     log(f"problem nt={n} ns={b} nb={a} m={prob.A.shape[0]} built in {time.perf_counter() - t0:.1f}s")
-    model = STGaussianModel(prob, device=dev, use_graph=not args.no_graph)
+    model = STGaussianModel(prob, device=dev, use_graph=args.graph)
     theta = prob.theta.copy()
     log("model resident on GPU")
 

@@ -114,7 +114,8 @@ double* bta_ws_logdet_ptr(bta_ws_t ws);
 #define BTA_OPT_FLOW_HELPERS 4
 #define BTA_OPT_FLOW_DELTA 5
 /* Batched factorization in the persistent form: problems factored
- * concurrently per wave (taken from the first problem's workspace; 0 = 6). */
+ * concurrently per wave (taken from the first problem's workspace; 0 = 6,
+ * at most 8). */
 #define BTA_OPT_BATCH_LANES 6
 int bta_ws_set_option(bta_ws_t ws, int option, int value);
 

@@ -22,8 +22,7 @@ struct bta_ws {
   bta::Workspace w;
   int use_graph = 0;
   int saved_use_graph = 0;
-  hipStream_t gstream = nullptr;  // capture / replay stream
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipStream_t gstream = nullptr;  // capture stream
   std::map<GraphKey, hipGraphExec_t> graphs;
   std::mutex mu;
 };
@@ -83,11 +82,10 @@ int run_maybe_graph(bta_ws* ws, const GraphKey& key, hipStream_t st, F&& enqueue
     if (e != hipSuccess) return (int)e;
     it = ws->graphs.emplace(key, exec).first;
   }
-  if ((e = hipEventRecord(ws->ev_in, st)) != hipSuccess) return (int)e;
-  if ((e = hipStreamWaitEvent(ws->gstream, ws->ev_in, 0)) != hipSuccess) return (int)e;
-  if ((e = hipGraphLaunch(it->second, ws->gstream)) != hipSuccess) return (int)e;
-  if ((e = hipEventRecord(ws->ev_out, ws->gstream)) != hipSuccess) return (int)e;
-  if ((e = hipStreamWaitEvent(st, ws->ev_out, 0)) != hipSuccess) return (int)e;
+  // Replayed on the caller's stream: solvers driven from different streams
+  // then run concurrently (a private replay stream per workspace can share a
+  // hardware queue with another workspace's, which serializes them).
+  if ((e = hipGraphLaunch(it->second, st)) != hipSuccess) return (int)e;
   return 0;
 }
 
@@ -144,8 +142,6 @@ int bta_ws_create(int device, int n, int b, int a, int use_graph, bta_ws_t* out)
   ws->use_graph = use_graph;
   if (use_graph) {
     hipStreamCreateWithFlags(&ws->gstream, hipStreamNonBlocking);
-    hipEventCreateWithFlags(&ws->ev_in, hipEventDisableTiming);
-    hipEventCreateWithFlags(&ws->ev_out, hipEventDisableTiming);
   }
   if ((e = hipDeviceSynchronize()) != hipSuccess) return (int)e;
   *out = ws;
@@ -159,8 +155,6 @@ int bta_ws_destroy(bta_ws_t ws) {
   if (ws->w.tlog) bta_ws_set_timing(ws, 0);
   for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);
   if (ws->gstream) hipStreamDestroy(ws->gstream);
-  if (ws->ev_in) hipEventDestroy(ws->ev_in);
-  if (ws->ev_out) hipEventDestroy(ws->ev_out);
   ws->w.trtri_plan.reset();
   ws->w.potri_plan.reset();
   ws->w.update_plan.reset();
@@ -239,8 +233,7 @@ struct bta_batch {
   std::vector<bta_ws*> ws;
   int device = 0;
   int use_graph = 0;
-  hipStream_t gstream = nullptr;
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipStream_t gstream = nullptr;  // capture stream
   hipGraphExec_t exec = nullptr;
 };
 
@@ -292,8 +285,6 @@ int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B
   bt->use_graph = use_graph;
   if (use_graph) {
     hipStreamCreateWithFlags(&bt->gstream, hipStreamNonBlocking);
-    hipEventCreateWithFlags(&bt->ev_in, hipEventDisableTiming);
-    hipEventCreateWithFlags(&bt->ev_out, hipEventDisableTiming);
   }
   *out = bt;
   return 0;
@@ -316,11 +307,7 @@ int bta_batch_potrf(bta_batch_t bt, void* stream) {
     hipGraphDestroy(graph);
     if (e != hipSuccess) return (int)e;
   }
-  if ((e = hipEventRecord(bt->ev_in, st)) != hipSuccess) return (int)e;
-  if ((e = hipStreamWaitEvent(bt->gstream, bt->ev_in, 0)) != hipSuccess) return (int)e;
-  if ((e = hipGraphLaunch(bt->exec, bt->gstream)) != hipSuccess) return (int)e;
-  if ((e = hipEventRecord(bt->ev_out, bt->gstream)) != hipSuccess) return (int)e;
-  if ((e = hipStreamWaitEvent(st, bt->ev_out, 0)) != hipSuccess) return (int)e;
+  if ((e = hipGraphLaunch(bt->exec, st)) != hipSuccess) return (int)e;
   return 0;
 }
 
@@ -330,8 +317,6 @@ int bta_batch_destroy(bta_batch_t bt) {
   hipDeviceSynchronize();
   if (bt->exec) hipGraphExecDestroy(bt->exec);
   if (bt->gstream) hipStreamDestroy(bt->gstream);
-  if (bt->ev_in) hipEventDestroy(bt->ev_in);
-  if (bt->ev_out) hipEventDestroy(bt->ev_out);
   delete bt;
   return 0;
 }

@@ -15,6 +15,15 @@
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+// Global-address-space views of pointers.  Pointers read from memory (job
+// tables) are generic to the compiler, which then emits FLAT accesses; a FLAT
+// load also counts against lgkmcnt, so every wait for an LDS read would wait
+// for the outstanding global prefetch as well.  Through these views the
+// compiler emits global_load / global_store.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ const gdouble* gview(const double* p) { return (const gdouble*)p; }
+__device__ __forceinline__ gdouble* gview(double* p) { return (gdouble*)p; }
+
 // Operand layout of v_mfma_f64_16x16x4f64 (D = A*B + C, A 16x4, B 4x16):
 //   A[i][k] is supplied by lane i + 16*k, B[k][j] by lane j + 16*k,
 //   D[i][j] is returned by lane j + 16*(i%4) in element i/4
@@ -84,6 +93,14 @@ __device__ __forceinline__ bool masked_out(int mask, int ig, int kg) {
   return (mask == MASK_K_GT_I && kg > ig) || (mask == MASK_K_LT_I && kg < ig);
 }
 
+// The mask a rows x kw chunk at global (i0, k0) needs: MASK_NONE when none of
+// its elements is masked, so that its loads need no per-element test.
+__device__ __forceinline__ int chunk_mask(int mask, int i0, int rows, int k0, int kw) {
+  if (mask == MASK_K_GT_I) return (k0 + kw - 1 <= i0) ? MASK_NONE : mask;
+  if (mask == MASK_K_LT_I) return (k0 >= i0 + rows - 1) ? MASK_NONE : mask;
+  return mask;
+}
+
 struct TilePrefetch {
   double r[16];
   // i0 / k0: global row / K index of the tile origin (used by the mask only)
@@ -91,17 +108,18 @@ struct TilePrefetch {
                                        int mask = MASK_NONE, int i0 = 0, int k0 = 0) {
     const int c = threadIdx.x & 63;
     const int r0 = threadIdx.x >> 6;
+    const gdouble* gg = gview(g);
     if (trans == 0) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int i = r0 + 4 * j;
-        r[j] = (i < rows && c < kw && !masked_out(mask, i0 + i, k0 + c)) ? g[(int64_t)i * ld + c] : 0.0;
+        r[j] = (i < rows && c < kw && !masked_out(mask, i0 + i, k0 + c)) ? gg[(int64_t)i * ld + c] : 0.0;
       }
     } else {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int k = r0 + 4 * j;
-        r[j] = (k < kw && c < rows && !masked_out(mask, i0 + c, k0 + k)) ? g[(int64_t)k * ld + c] : 0.0;
+        r[j] = (k < kw && c < rows && !masked_out(mask, i0 + c, k0 + k)) ? gg[(int64_t)k * ld + c] : 0.0;
       }
     }
   }
@@ -148,7 +166,7 @@ __device__ __forceinline__ void acc_load(TileAcc& t, const double* __restrict__ 
                                          int rows, int cols) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int rb = (w >> 1) * 32 + (lane >> 4), cb = (w & 1) * 32 + (lane & 15);
-  const double* base = g + (int64_t)rb * ld + cb;
+  const gdouble* base = gview(g) + (int64_t)rb * ld + cb;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -164,7 +182,7 @@ __device__ __forceinline__ void acc_store(const TileAcc& t, double* __restrict__
                                           int rows, int cols) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int rb = (w >> 1) * 32 + (lane >> 4), cb = (w & 1) * 32 + (lane & 15);
-  double* base = g + (int64_t)rb * ld + cb;
+  gdouble* base = gview(g) + (int64_t)rb * ld + cb;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -181,7 +199,7 @@ __device__ __forceinline__ void acc_store_lower(const TileAcc& t, double* __rest
                                                 int rows, int cols) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int rb = (w >> 1) * 32 + (lane >> 4), cb = (w & 1) * 32 + (lane & 15);
-  double* base = g + (int64_t)rb * ld + cb;
+  gdouble* base = gview(g) + (int64_t)rb * ld + cb;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll

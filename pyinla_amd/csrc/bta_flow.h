@@ -112,6 +112,16 @@ struct FlowArgs {
   int helpers;               // helper workgroups of this launch entry
 };
 
+// Entries of one batched launch, passed by value: pointers in kernel
+// arguments are known to the compiler to be global (read from a table in
+// memory they would be generic and every access a FLAT one).
+#define FLOW_BATCH_MAX 8
+struct FlowBatch {
+  FlowArgs e[FLOW_BATCH_MAX];
+  int begin[FLOW_BATCH_MAX + 1];  // entry q owns workgroups [begin[q], begin[q+1])
+  int nent;
+};
+
 int flow_owner(const FlowShape& s, int rho, int c, int helpers);
 // delta >= 1: level order, far tiles' updates grouped by delta columns;
 // delta == 0: list order from a simulated launch.  False on an internal

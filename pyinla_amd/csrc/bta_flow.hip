@@ -584,9 +584,11 @@ __device__ __forceinline__ HStep make_step(const FlowArgs& p, int t, int kk) {
   return st;
 }
 
-__device__ __forceinline__ bool flag_is(const unsigned long long* f, unsigned long long tag) {
-  return f == nullptr || __hip_atomic_load(const_cast<unsigned long long*>(f), __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_SYSTEM) == tag;
+// Current value of a flag (`tag` for an absent flag).
+__device__ __forceinline__ unsigned long long flag_value(const unsigned long long* f, unsigned long long tag) {
+  return f == nullptr ? tag
+                      : __hip_atomic_load(const_cast<unsigned long long*>(f), __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Issue the operand loads of a step into registers.  The exchange tiles and
@@ -626,6 +628,19 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
       if (cur.f2 && !flow_wait(p, cur.f2, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
       step_fetch(cur, pa, pb);
     }
+    // the following step.  Thread 0 issues the loads of its flags now, both
+    // at once, so that their latency overlaps the staging of this step.
+    HStep nxt;
+    bool has_next = true;
+    if (cur.op == FLOW_NEXT && cur.kk + 1 < min(cur.k + FLOW_SPAN, nT)) nxt = make_step(p, cur.t, cur.kk + 1);
+    else if (cur.t + 1 < t1) nxt = make_step(p, cur.t + 1, -1);
+    else has_next = false;
+    const bool probe = has_next && nxt.op != FLOW_ZERO;
+    unsigned long long fv1 = 0, fv2 = 0;
+    if (threadIdx.x == 0 && probe) {
+      fv1 = flag_value(nxt.f1, tag);
+      fv2 = flag_value(nxt.f2, tag);
+    }
     // accumulator of the step's target
     double* tg = g;
     int64_t ld = p.b;
@@ -657,14 +672,8 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
     if (cur.op == FLOW_TRSM) acc_to_lds(acc, sA);
     else pa.store(sA, 0);
     if (cur.srcB != cur.srcA) pb.store(sB, 0);
-    // the following step; its operands are fetched now if they are ready
-    HStep nxt;
-    bool has_next = true;
-    if (cur.op == FLOW_NEXT && cur.kk + 1 < min(cur.k + FLOW_SPAN, nT)) nxt = make_step(p, cur.t, cur.kk + 1);
-    else if (cur.t + 1 < t1) nxt = make_step(p, cur.t + 1, -1);
-    else has_next = false;
-    if (threadIdx.x == 0)
-      *ok_sh = (has_next && nxt.op != FLOW_ZERO && flag_is(nxt.f1, tag) && flag_is(nxt.f2, tag)) ? 1 : 0;
+    // the following step's operands are fetched now if they are ready
+    if (threadIdx.x == 0) *ok_sh = (probe && fv1 == tag && fv2 == tag) ? 1 : 0;
     __syncthreads();
     fetched = (*ok_sh != 0);
     if (fetched) step_fetch(nxt, pa, pb);
@@ -834,27 +843,26 @@ void flow_launch_block(const Workspace& ws, const FlowPlan& fp, const FlowSched&
 // Batched form: entry e of a launch owns workgroups [begin[e], begin[e+1]):
 // its chain first, then its helpers.  All entries' workgroups are resident
 // together (their sum is within the device's capacity).
-__global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow_batched(const FlowArgs* __restrict__ args,
-                                                                       const int* __restrict__ begin, int nent) {
+__global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow_batched(FlowBatch fb) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
   __shared__ double sScr[4 * 64 + 64];
   __shared__ int ok_sh;
-  int lo = 0, hi = nent - 1;
+  int lo = 0, hi = fb.nent - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (begin[mid] <= (int)blockIdx.x) lo = mid;
+    if (fb.begin[mid] <= (int)blockIdx.x) lo = mid;
     else hi = mid - 1;
   }
-  const FlowArgs p = args[lo];
-  const int local = (int)blockIdx.x - begin[lo];
+  const FlowArgs p = fb.e[lo];
+  const int local = (int)blockIdx.x - fb.begin[lo];
   const unsigned long long tag = (p.epoch[0] << 20) | (unsigned long long)p.blk;
   if (local == 0) flow_chain(p, tag, sA, sB, sScr, &ok_sh);
   else flow_helper(p, local - 1, tag, sA, sB, &ok_sh);
 }
 
-void flow_launch_batched(const FlowArgs* d_args, const int* d_begin, int nent, int total, hipStream_t st) {
-  hipLaunchKernelGGL(k_panel_flow_batched, dim3(total), dim3(BTA_THREADS), 0, st, d_args, d_begin, nent);
+void flow_launch_batched(const FlowBatch& fb, int total, hipStream_t st) {
+  hipLaunchKernelGGL(k_panel_flow_batched, dim3(total), dim3(BTA_THREADS), 0, st, fb);
 }
 
 int flow_capacity(int device) {

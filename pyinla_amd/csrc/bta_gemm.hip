@@ -17,9 +17,11 @@ namespace bta {
 #endif
 
 // acc += alpha * sA(64 x K) * sB(64 x K)^T over k in [0, kend) (LDS stride LD).
-template <int LD = BTA_LDS_LD>
+// KFIX > 0: kend == KFIX is known at compile time (fully unrolled).
+template <int LD = BTA_LDS_LD, int KFIX = 0>
 __device__ __forceinline__ void mma_nt_alpha(TileAcc& t, const double* __restrict__ sA,
                                              const double* __restrict__ sB, int kend, double alpha) {
+  if (KFIX > 0) kend = KFIX;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
@@ -60,8 +62,22 @@ __device__ __forceinline__ void decode_lower(int idx, int& tr, int& tc) {
 struct ChunkPrefetch {
   static constexpr int N = 64 * BTA_GEMM_KC / BTA_THREADS;
   double r[N];
-  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw,
+  __device__ __forceinline__ void load(const double* __restrict__ g0, int64_t ld, int trans, int rows, int kw,
                                        int mask, int i0, int k0) {
+    const gdouble* g = gview(g0);
+    mask = chunk_mask(mask, i0, rows, k0, kw);
+    if (mask == MASK_NONE && rows >= 64 && kw >= BTA_GEMM_KC) {  // interior chunk: no tests
+      if (trans == 0) {
+        const gdouble* t = g + (int64_t)(threadIdx.x / BTA_GEMM_KC) * ld + threadIdx.x % BTA_GEMM_KC;
+#pragma unroll
+        for (int q = 0; q < N; ++q) r[q] = t[(int64_t)q * (BTA_THREADS / BTA_GEMM_KC) * ld];
+      } else {
+        const gdouble* t = g + (int64_t)(threadIdx.x >> 6) * ld + (threadIdx.x & 63);
+#pragma unroll
+        for (int q = 0; q < N; ++q) r[q] = t[(int64_t)q * (BTA_THREADS / 64) * ld];
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < N; ++q) {
       const int e = threadIdx.x + BTA_THREADS * q;
@@ -164,7 +180,11 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
     __syncthreads();
     have = next_chunk(false);
     if (have) fetch();
-    mma_nt_alpha<BTA_GEMM_LD>(acc, sA, sB, (kw + 3) & ~3, alpha);
+#ifdef BTA_GEMM_UNROLL
+    if (kw == BTA_GEMM_KC) mma_nt_alpha<BTA_GEMM_LD, BTA_GEMM_KC>(acc, sA, sB, BTA_GEMM_KC, alpha);
+    else
+#endif
+      mma_nt_alpha<BTA_GEMM_LD>(acc, sA, sB, (kw + 3) & ~3, alpha);
     __syncthreads();
   }
   // epilogue
@@ -177,14 +197,15 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
         const int i = acc_row(x, r), j = acc_col(y);
         if (i >= rows || j >= cols) continue;
         const double v = acc.c[x][y][r];
+        gdouble* cg = gview(ct);
         if (g.lower && tr == tc) {
           if (i >= j) {
-            ct[(int64_t)i * g.ldc + j] = v;
-            if (g.mirror && i > j) ct[(int64_t)j * g.ldc + i] = v;
+            cg[(int64_t)i * g.ldc + j] = v;
+            if (g.mirror && i > j) cg[(int64_t)j * g.ldc + i] = v;
           }
         } else {
-          ct[(int64_t)i * g.ldc + j] = v;
-          if (g.lower && g.mirror) g.C[(int64_t)(tc * 64 + j) * g.ldc + tr * 64 + i] = v;
+          cg[(int64_t)i * g.ldc + j] = v;
+          if (g.lower && g.mirror) gview(g.C)[(int64_t)(tc * 64 + j) * g.ldc + tr * 64 + i] = v;
         }
       }
 }
@@ -209,8 +230,22 @@ struct BigOp {
   double r[N];
   // trans == 0: X row-major rows x K -> element (i, k) = X[i][k]
   // trans == 1: X row-major K x rows -> element (i, k) = X[k][i]
-  __device__ __forceinline__ void load(const double* __restrict__ g, int64_t ld, int trans, int rows, int kw,
+  __device__ __forceinline__ void load(const double* __restrict__ g0, int64_t ld, int trans, int rows, int kw,
                                        int mask, int i0, int k0) {
+    const gdouble* g = gview(g0);
+    mask = chunk_mask(mask, i0, rows, k0, kw);
+    if (mask == MASK_NONE && rows >= BIG_TS && kw >= BIG_KC) {  // interior chunk: no tests
+      if (trans == 0) {
+        const gdouble* t = g + (int64_t)(threadIdx.x / BIG_KC) * ld + threadIdx.x % BIG_KC;
+#pragma unroll
+        for (int p = 0; p < N; ++p) r[p] = t[(int64_t)p * (BTA_THREADS / BIG_KC) * ld];
+      } else {
+        const gdouble* t = g + (int64_t)(threadIdx.x >> 7) * ld + (threadIdx.x & 127);
+#pragma unroll
+        for (int p = 0; p < N; ++p) r[p] = t[(int64_t)p * (BTA_THREADS / BIG_TS) * ld];
+      }
+      return;
+    }
 #pragma unroll
     for (int p = 0; p < N; ++p) {
       const int e = threadIdx.x + BTA_THREADS * p;
@@ -233,10 +268,12 @@ struct BigOp {
   }
 };
 
-__global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped_big(const GemmJob* __restrict__ jobs,
-                                                                 const int* __restrict__ prefix, int njobs) {
-  __shared__ double sA[BIG_TS * BIG_LD];
-  __shared__ double sB[BIG_TS * BIG_LD];
+__global__ void __launch_bounds__(BTA_THREADS, 1) k_gemm_grouped_big(const GemmJob* __restrict__ jobs,
+                                                                    const int* __restrict__ prefix, int njobs) {
+  // two LDS buffers per operand: chunk q is multiplied out of one while chunk
+  // q+1, fetched into registers before the multiply, is stored to the other
+  __shared__ double sA[2][BIG_TS * BIG_LD];
+  __shared__ double sB[2][BIG_TS * BIG_LD];
   const int jid = find_job(prefix, njobs, blockIdx.x);
   const GemmJob& g = jobs[jid];
   const int local = blockIdx.x - prefix[jid];
@@ -261,7 +298,7 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped_big(const GemmJob*
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int i = wm + 16 * x + 4 * r + lk, j = wn + 16 * y + li;
-          acc[x][y][r] = (i < rows && j < cols) ? g.beta * ct[(int64_t)i * g.ldc + j] : 0.0;
+          acc[x][y][r] = (i < rows && j < cols) ? g.beta * gview(ct)[(int64_t)i * g.ldc + j] : 0.0;
         }
       } else {
         acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
@@ -306,29 +343,45 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped_big(const GemmJob*
     else pb.load(sg.B + (int64_t)k0 * sg.ldb + tc * BIG_TS, sg.ldb, 1, cols, kw, mb, tc * BIG_TS, k0);
   };
   bool have = (g.nseg > 0) && next_chunk(true);
-  if (have) fetch();
-  while (have) {
+  int buf = 0;
+  if (have) {
+    fetch();
     const GemmSeg& sg = g.seg[seg_id];
+    pa.store(sA[0], sg.ta);
+    pb.store(sB[0], (sg.tb == 1) ? 0 : 1);
+    __syncthreads();
+  }
+  while (have) {
     const int kw = min(BIG_KC, kmax - k0);
-    const double alpha = sg.alpha;
-    const int sta = sg.ta, stb = (sg.tb == 1) ? 0 : 1;
-    pa.store(sA, sta);
-    pb.store(sB, stb);
-    __syncthreads();
-    have = next_chunk(false);
-    if (have) fetch();
-    for (int k = 0; k < kw; k += 4) {
-      double av[4], bv[4];
+    const double alpha = g.seg[seg_id].alpha;
+    const bool more = next_chunk(false);
+    if (more) fetch();
+    const double* cA = sA[buf];
+    const double* cB = sB[buf];
 #pragma unroll
-      for (int x = 0; x < 4; ++x) av[x] = alpha * sA[(wm + 16 * x + li) * BIG_LD + k + lk];
+    for (int k = 0; k < BIG_KC; k += 4) {
+      if (k < kw) {
+        double av[4], bv[4];
 #pragma unroll
-      for (int y = 0; y < 4; ++y) bv[y] = sB[(wn + 16 * y + li) * BIG_LD + k + lk];
+        for (int x = 0; x < 4; ++x) av[x] = alpha * cA[(wm + 16 * x + li) * BIG_LD + k + lk];
 #pragma unroll
-      for (int x = 0; x < 4; ++x)
+        for (int y = 0; y < 4; ++y) bv[y] = cB[(wn + 16 * y + li) * BIG_LD + k + lk];
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = mfma_f64(av[x], bv[y], acc[x][y]);
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) acc[x][y] = mfma_f64(av[x], bv[y], acc[x][y]);
+      }
     }
+    if (more) {
+      const GemmSeg& sg = g.seg[seg_id];
+      pa.store(sA[buf ^ 1], sg.ta);
+      pb.store(sB[buf ^ 1], (sg.tb == 1) ? 0 : 1);
+    }
+    // one barrier per chunk: the buffer stored above was last read before
+    // the previous barrier, the one read next was stored before this one
     __syncthreads();
+    buf ^= 1;
+    have = more;
   }
 #pragma unroll
   for (int x = 0; x < 4; ++x)
@@ -339,14 +392,15 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemm_grouped_big(const GemmJob*
         const int i = wm + 16 * x + 4 * r + lk, j = wn + 16 * y + li;
         if (i >= rows || j >= cols) continue;
         const double v = acc[x][y][r];
+        gdouble* cg = gview(ct);
         if (g.lower && tr == tc) {
           if (i >= j) {
-            ct[(int64_t)i * g.ldc + j] = v;
-            if (g.mirror && i > j) ct[(int64_t)j * g.ldc + i] = v;
+            cg[(int64_t)i * g.ldc + j] = v;
+            if (g.mirror && i > j) cg[(int64_t)j * g.ldc + i] = v;
           }
         } else {
-          ct[(int64_t)i * g.ldc + j] = v;
-          if (g.lower && g.mirror) g.C[(int64_t)(tc * BIG_TS + j) * g.ldc + tr * BIG_TS + i] = v;
+          cg[(int64_t)i * g.ldc + j] = v;
+          if (g.lower && g.mirror) gview(g.C)[(int64_t)(tc * BIG_TS + j) * g.ldc + tr * BIG_TS + i] = v;
         }
       }
 }

@@ -67,7 +67,7 @@ int flow_capacity(int device);  // co-resident panel workgroups on the device
 FlowArgs flow_args(const FlowPlan& fp, const FlowSched& sc, double* D, double* L, double* B, double* T, double* winv,
                    double* logpart, const unsigned long long* epoch, unsigned long long* info, int* err, int i,
                    int n, int b, int a);
-void flow_launch_batched(const FlowArgs* d_args, const int* d_begin, int nent, int total, hipStream_t st);
+void flow_launch_batched(const FlowBatch& fb, int total, hipStream_t st);
 unsigned long long flow_spin_ticks(int device, double seconds);
 
 // Per-solver device workspace.  Allocated once (never inside a hot call) for a
@@ -152,7 +152,7 @@ struct ProbDesc {
 // the last column of step i one grouped update launch.
 struct BatchPlan {
   struct Launch { int i, k, nent, total, off_prob, off_begin, update; };
-  struct FLaunch { int nent, total, off_args, off_begin; };
+  struct FLaunch { FlowBatch fb; int total; };
   std::vector<ProbDesc> probs;
   std::vector<Launch> launches;
   ProbDesc* d_probs = nullptr;
@@ -162,13 +162,12 @@ struct BatchPlan {
   bool flow = false;
   std::vector<std::unique_ptr<FlowPlan>> fplans;
   std::vector<FLaunch> flaunches;
-  FlowArgs* d_fargs = nullptr;
-  int* d_fbegin = nullptr;
   BatchPlan() = default;
   BatchPlan(const BatchPlan&) = delete;
   BatchPlan& operator=(const BatchPlan&) = delete;
   ~BatchPlan();
-  // lanes: problems factored concurrently in the persistent form (0 = 6)
+  // lanes: problems factored concurrently in the persistent form (0 = 6,
+  // at most FLOW_BATCH_MAX)
   int build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int delta, int device);
   int enqueue(hipStream_t st) const;
 };

@@ -559,8 +559,6 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
 BatchPlan::~BatchPlan() {
   if (d_probs) hipFree(d_probs);
   if (d_ent) hipFree(d_ent);
-  if (d_fargs) hipFree(d_fargs);
-  if (d_fbegin) hipFree(d_fbegin);
 }
 
 int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int delta, int device) {
@@ -576,12 +574,10 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
     // the problems still active at each block step
     const int cap = flow_capacity(device);
     const int np = (int)ps.size();
-    const int nl = lanes > 0 ? std::min(lanes, np) : std::min(np, 6);
+    const int nl = std::min(lanes > 0 ? std::min(lanes, np) : std::min(np, 6), FLOW_BATCH_MAX);
     if (cap / nl - 1 < 1) {
       flow = false;  // too many problems to give each a resident chain + helper
     } else {
-      std::vector<FlowArgs> args;
-      std::vector<int> begin;
       int nmax = 0;
       for (const auto& d : ps) nmax = std::max(nmax, d.n);
       fplans.clear();
@@ -606,8 +602,7 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
           for (int t = w0; t < w1; ++t) nact += (i < ps[order[t]].n) ? 1 : 0;
           const int helpers = std::min(1023, cap / nact - 1);
           FLaunch L;
-          L.off_args = (int)args.size();
-          L.off_begin = (int)begin.size();
+          L.fb.nent = 0;
           int total = 0;
           for (int t = w0; t < w1; ++t) {
             const int q = order[t];
@@ -616,27 +611,18 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
             const FlowSched* sc = nullptr;
             const int rc = fplans[q]->sched(helpers, delta, &sc);
             if (rc) return rc;
-            args.push_back(flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch, d.info,
-                                     d.err, i, d.n, d.b, d.a));
-            begin.push_back(total);
+            L.fb.e[L.fb.nent] = flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch,
+                                          d.info, d.err, i, d.n, d.b, d.a);
+            L.fb.begin[L.fb.nent++] = total;
             total += 1 + helpers;
           }
-          begin.push_back(total);
-          L.nent = (int)args.size() - L.off_args;
+          L.fb.begin[L.fb.nent] = total;
           L.total = total;
           flaunches.push_back(L);
         }
       }
       (void)nmax;
       hipError_t e;
-      if ((e = hipMalloc(&d_fargs, args.size() * sizeof(FlowArgs))) != hipSuccess) return (int)e;
-      if ((e = hipMemcpy(d_fargs, args.data(), args.size() * sizeof(FlowArgs), hipMemcpyHostToDevice)) !=
-          hipSuccess)
-        return (int)e;
-      if ((e = hipMalloc(&d_fbegin, begin.size() * sizeof(int))) != hipSuccess) return (int)e;
-      if ((e = hipMemcpy(d_fbegin, begin.data(), begin.size() * sizeof(int), hipMemcpyHostToDevice)) !=
-          hipSuccess)
-        return (int)e;
       if ((e = hipMalloc(&d_probs, ps.size() * sizeof(ProbDesc))) != hipSuccess) return (int)e;
       if ((e = hipMemcpy(d_probs, ps.data(), ps.size() * sizeof(ProbDesc), hipMemcpyHostToDevice)) != hipSuccess)
         return (int)e;
@@ -721,7 +707,7 @@ int BatchPlan::enqueue(hipStream_t st) const {
   hipLaunchKernelGGL(k_begin_batch, dim3((np + 63) / 64), dim3(64), 0, st, d_probs, np);
   if (flow) {  // block updates are folded into the panel launches
     for (const FLaunch& L : flaunches)
-      flow_launch_batched(d_fargs + L.off_args, d_fbegin + L.off_begin, L.nent, L.total, st);
+      flow_launch_batched(L.fb, L.total, st);
     hipLaunchKernelGGL(k_tip_potrf_batched, dim3(np), dim3(BTA_THREADS), 0, st, d_probs);
     hipLaunchKernelGGL(k_logdet_reduce_batched, dim3(np), dim3(BTA_THREADS), 0, st, d_probs);
     return (int)hipGetLastError();

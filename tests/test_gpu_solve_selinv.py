@@ -9,6 +9,7 @@ import pytest
 import torch
 
 from oracle import bta as ob
+from pyinla_amd import _lib
 from pyinla_amd.solver import SerinvSolver
 from pyinla_amd.workloads import known_factor_bta
 
@@ -183,3 +184,34 @@ def test_batched_flow_factorization(cuda, force_flow):
             if B is not None:
                 assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
                 assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
+
+
+@pytest.mark.parametrize("n,b,a", [(3, 600, 5), (2, 130, 0), (4, 92, 4), (2, 300, 6)])
+def test_selected_inversion_with_128_tiles(cuda, n, b, a):
+    """The 128x128-tile grouped GEMM (block inverses + selected inversion,
+    ragged edges, triangular operands, lower/mirror outputs) matches the
+    oracle."""
+    kf = known_factor_bta(n, b, a, seed=400 + n * b + a)
+    s = SerinvSolver(None, b, n, a, use_graph=False)
+    _lib.check(s.lib.bta_ws_set_option(s._ws, 2, 128), "set_option")
+    s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if a else None, kf.Q_T if a else None)
+    s.factorize("bta")
+    rng = np.random.default_rng(3)
+    R = rng.normal(size=(n * b + a, 2))
+    xr = R.copy()
+    ob.pobtas(kf.Lf_D, kf.Lf_L, kf.Lf_B if a else None, kf.Lf_T if a else None, xr, "N")
+    ob.pobtas(kf.Lf_D, kf.Lf_L, kf.Lf_B if a else None, kf.Lf_T if a else None, xr, "C")
+    x = torch.from_numpy(R.copy()).to(cuda)
+    s.solve(x, "bta")
+    assert _rel(x.cpu().numpy(), xr) < RTOL
+    s.selected_inversion("bta")
+    D, L = kf.Lf_D.copy(), kf.Lf_L.copy()
+    B = kf.Lf_B.copy() if a else None
+    T = kf.Lf_T.copy() if a else None
+    ob.pobtasi(D, L, B, T)
+    assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
+    if n > 1:
+        assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < RTOL
+    if a:
+        assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
+        assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
