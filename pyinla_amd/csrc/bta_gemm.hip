@@ -49,6 +49,28 @@ __device__ __forceinline__ int find_job(const int* __restrict__ prefix, int njob
   return lo;
 }
 
+// Output tile of workgroup `local` of a tm x tn job.  Workgroups are
+// dispatched to the 8 XCDs round-robin (blockIdx % 8); when the job's tiles
+// start on an XCD boundary and split evenly, XCD x takes a contiguous run of
+// the tiles ordered in bands of 8 tile rows (column by column inside a band),
+// i.e. a compact 8 x (T/64) rectangle whose A and B panels its L2 shares.
+#ifndef BTA_GEMM_XCD
+#define BTA_GEMM_XCD 1
+#endif
+__device__ __forceinline__ void xcd_tile(int local, int first, int tm, int tn, int& tr, int& tc) {
+  const int T = tm * tn;
+  int t = local;
+  if (BTA_GEMM_XCD && (first & 7) == 0 && (T & 7) == 0 && tm % 8 == 0) t = (local & 7) * (T >> 3) + (local >> 3);
+  if (BTA_GEMM_XCD && tm % 8 == 0) {
+    const int band = t / (8 * tn), idx = t % (8 * tn);
+    tr = band * 8 + (idx & 7);
+    tc = idx >> 3;
+  } else {
+    tr = t / tn;
+    tc = t % tn;
+  }
+}
+
 __device__ __forceinline__ void decode_lower(int idx, int& tr, int& tc) {
   int r = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
   while ((r + 1) * (r + 2) / 2 <= idx) ++r;
@@ -110,9 +132,8 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
   int tr, tc;
   if (g.lower) decode_lower(local, tr, tc);
   else {
-    const int tn = (g.n + 63) / 64;
-    tr = local / tn;
-    tc = local % tn;
+    const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
+    xcd_tile(local, prefix[jid], tm, tn, tr, tc);
   }
   const int rows = min(64, g.m - tr * 64), cols = min(64, g.n - tc * 64);
   double* ct = g.C + (int64_t)(tr * 64) * g.ldc + tc * 64;
