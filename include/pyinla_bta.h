@@ -144,7 +144,7 @@ int bta_ws_get_timing(bta_ws_t ws, int kclass, double* total_ms, int64_t* count)
  * trans == 1 solves LL^T x = rhs (serinv.pobtas(..., trans="C")); the pair
  * replaces SerinvSolver.solve (structured_solver.py:102-149).  a == 0 gives
  * pobts.  rhs is (n*b + a) x nrhs, row-major, leading dimension ldrhs
- * (ldrhs >= nrhs, b*nrhs <= 3*b*b).  The first call after a factorization
+ * (ldrhs >= nrhs, any nrhs).  The first call after a factorization
  * also forms W_i = D_i^-1 for every block (kept in the workspace). */
 int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
               double* rhs, int nrhs, int ldrhs, int trans, void* stream);

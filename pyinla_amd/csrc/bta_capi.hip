@@ -197,7 +197,6 @@ int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
   int rc = check_shape(ws, n, b, a);
   if (rc) return rc;
   if (!D || (n > 1 && !L) || (a > 0 && (!B || !T)) || !rhs || nrhs < 1 || ldrhs < nrhs) return BTA_E_ARG;
-  if ((size_t)b * nrhs > ws->w.work_doubles) return BTA_E_ARG;
   if (trans != 0 && trans != 1) return BTA_E_ARG;
   hipStream_t st = as_stream(stream);
   rc = ensure_inverses(ws, D, n, b, st);

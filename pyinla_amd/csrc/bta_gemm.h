@@ -141,6 +141,9 @@ struct DevPlan {
   bool empty() const { return launches.empty(); }
 };
 
-void gemv_launch(const GemvJob& g, int form_t, hipStream_t st);
+// form_t = 1 (op(A) = A^T) needs gemv_scratch(g, 1) doubles of scratch; y may
+// then alias an x of the job (the partial sums are complete before y is written).
+size_t gemv_scratch(const GemvJob& g, int form_t);
+void gemv_launch(const GemvJob& g, int form_t, double* scratch, hipStream_t st);
 
 }  // namespace bta

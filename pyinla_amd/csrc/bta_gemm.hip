@@ -428,9 +428,15 @@ __global__ void __launch_bounds__(BTA_THREADS, 1) k_gemm_grouped_big(const GemmJ
 
 // ------------------------------------------------------------------- GEMV
 //
-// N form (op(A) = A): one wave per output row, lanes stride the columns,
-// butterfly reduction.  T form (op(A) = A^T): 16 output columns per
-// workgroup, 16 row groups sum interleaved rows, fixed-order combine.
+// Memory-bound sweeps: every matrix element is read once, so the kernels are
+// shaped for bandwidth.  N form (op(A) = A): one wave per output row; with a
+// single unit-stride right-hand side each lane reads two consecutive doubles
+// per access, four accesses in flight, and a butterfly combines the lanes.
+// T form (op(A) = A^T): two passes.  Workgroup (64-column block, K chunk of
+// 128 rows) sums its rows for 64 columns (one per lane, the four waves take
+// every fourth row, four rows in flight) into a partial vector; a second
+// kernel adds the partials of each column in chunk order into y.  Both
+// orders of summation are fixed, so results are reproducible run to run.
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -438,62 +444,109 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// blockIdx.x covers rows in groups of 4 (N form) or columns in groups of 64 (T form).
-__global__ void __launch_bounds__(BTA_THREADS) k_gemv(GemvJob g, int form_t) {
+#define GEMV_TK 128  // rows of A per T-form chunk
+
+__global__ void __launch_bounds__(BTA_THREADS) k_gemv_n(GemvJob g) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (!form_t) {
-    const int i = blockIdx.x * 4 + w;
-    if (i >= g.m) return;
-    for (int c = 0; c < g.nrhs; ++c) {
-      double acc = 0.0;
-      for (int s = 0; s < g.nseg; ++s) {
-        const GemvSeg& sg = g.seg[s];
-        int kmin = 0, kmax = sg.k;
-        if (sg.tri == 1) kmax = min(kmax, i + 1);
-        if (sg.tri == 2) kmin = i;
-        const double* a = sg.A + (int64_t)i * sg.lda;
-        double part = 0.0;
-        for (int kk = kmin + lane; kk < kmax; kk += 64) part = fma(a[kk], sg.x[(int64_t)kk * sg.ldx + c], part);
-        acc = fma(sg.alpha, wave_sum(part), acc);
+  const int i = blockIdx.x * 4 + w;
+  if (i >= g.m) return;
+  for (int c = 0; c < g.nrhs; ++c) {
+    double acc = 0.0;
+    for (int s = 0; s < g.nseg; ++s) {
+      const GemvSeg& sg = g.seg[s];
+      int kmin = 0, kmax = sg.k;
+      if (sg.tri == 1) kmax = min(kmax, i + 1);
+      if (sg.tri == 2) kmin = i;
+      const double* a = sg.A + (int64_t)i * sg.lda;
+      double p[4] = {0.0, 0.0, 0.0, 0.0};
+      const bool vec = sg.ldx == 1 && g.nrhs == 1 && (sg.lda & 1) == 0 &&
+                       ((reinterpret_cast<uintptr_t>(sg.A) | reinterpret_cast<uintptr_t>(sg.x)) & 15) == 0;
+      if (vec) {
+        const int k0 = kmin & ~1;
+        for (int base = k0; base < kmax; base += 512) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int kk = base + 128 * u + 2 * lane;
+            if (kk + 1 < kmax) {
+              const double2 av = *reinterpret_cast<const double2*>(a + kk);
+              const double2 xv = *reinterpret_cast<const double2*>(sg.x + kk);
+              p[u] = fma(av.y, xv.y, p[u]);
+              if (kk >= kmin) p[u] = fma(av.x, xv.x, p[u]);
+            } else if (kk < kmax && kk >= kmin) {
+              p[u] = fma(a[kk], sg.x[kk], p[u]);
+            }
+          }
+        }
+      } else {
+        for (int base = kmin; base < kmax; base += 256) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int kk = base + 64 * u + lane;
+            if (kk < kmax) p[u] = fma(a[kk], sg.x[(int64_t)kk * sg.ldx + c], p[u]);
+          }
+        }
       }
-      if (lane == 0) {
-        double* yp = g.y + (int64_t)i * g.ldy + c;
-        *yp = (g.beta != 0.0) ? fma(g.beta, *yp, acc) : acc;
-      }
+      acc = fma(sg.alpha, wave_sum((p[0] + p[1]) + (p[2] + p[3])), acc);
     }
-  } else {
-    // 16 output columns per workgroup; thread (c = t%16, g = t/16) sums rows
-    // k = g, g+16, ... of column j, then a fixed-order combine over g.
-    __shared__ double red[16][17];
-    const int c = threadIdx.x & 15, grp = threadIdx.x >> 4;
-    const int j = blockIdx.x * 16 + c;
-    for (int rc = 0; rc < g.nrhs; ++rc) {
-      double acc = 0.0;
-      for (int s = 0; s < g.nseg; ++s) {
-        const GemvSeg& sg = g.seg[s];
-        double part = 0.0;
-        if (j < g.m) {
-          int kmin = 0, kmax = sg.k;
-          if (sg.tri == 1) kmax = min(kmax, j + 1);
-          if (sg.tri == 2) kmin = j;
-          for (int kk = kmin + grp; kk < kmax; kk += 16)
-            part = fma(sg.A[(int64_t)kk * sg.lda + j], sg.x[(int64_t)kk * sg.ldx + rc], part);
-        }
-        red[grp][c] = part;
-        __syncthreads();
-        if (grp == 0) {
-          double t = 0.0;
-          for (int q = 0; q < 16; ++q) t += red[q][c];
-          acc = fma(sg.alpha, t, acc);
-        }
-        __syncthreads();
-      }
-      if (grp == 0 && j < g.m) {
-        double* yp = g.y + (int64_t)j * g.ldy + rc;
-        *yp = (g.beta != 0.0) ? fma(g.beta, *yp, acc) : acc;
-      }
+    if (lane == 0) {
+      double* yp = g.y + (int64_t)i * g.ldy + c;
+      *yp = (g.beta != 0.0) ? fma(g.beta, *yp, acc) : acc;
     }
   }
+}
+
+__host__ __device__ inline int gemv_t_chunks(const GemvJob& g) {
+  int n = 0;
+  for (int s = 0; s < g.nseg; ++s) n += (g.seg[s].k + GEMV_TK - 1) / GEMV_TK;
+  return n;
+}
+
+// part[(q * m + j) * nrhs + rc]: alpha_s * sum over chunk q's rows of A[k][j] x[k].
+__global__ void __launch_bounds__(BTA_THREADS) k_gemv_t_part(GemvJob g, double* __restrict__ part) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + lane;
+  int q = blockIdx.y, s = 0;
+  while (s < g.nseg - 1) {
+    const int nc = (g.seg[s].k + GEMV_TK - 1) / GEMV_TK;
+    if (q < nc) break;
+    q -= nc;
+    ++s;
+  }
+  const GemvSeg& sg = g.seg[s];
+  const int kb = q * GEMV_TK, ke = min(sg.k, kb + GEMV_TK);
+  int klo = kb, khi = ke;  // this lane's rows (triangular operands)
+  if (sg.tri == 1) khi = min(khi, j + 1);
+  if (sg.tri == 2) klo = max(klo, j);
+  const bool col = j < g.m;
+  for (int rc = 0; rc < g.nrhs; ++rc) {
+    double p[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int base = kb + w; base < ke; base += 16) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = base + 4 * u;
+        if (col && k < ke && k >= klo && k < khi)
+          p[u] = fma(sg.A[(int64_t)k * sg.lda + j], sg.x[(int64_t)k * sg.ldx + rc], p[u]);
+      }
+    }
+    red[w][lane] = (p[0] + p[1]) + (p[2] + p[3]);
+    __syncthreads();
+    if (w == 0 && col)
+      part[((int64_t)blockIdx.y * g.m + j) * g.nrhs + rc] =
+          sg.alpha * ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(BTA_THREADS) k_gemv_t_reduce(GemvJob g, const double* __restrict__ part,
+                                                              int nchunk) {
+  const int64_t e = (int64_t)blockIdx.x * BTA_THREADS + threadIdx.x;
+  if (e >= (int64_t)g.m * g.nrhs) return;
+  const int j = (int)(e / g.nrhs), rc = (int)(e % g.nrhs);
+  double acc = 0.0;
+  for (int q = 0; q < nchunk; ++q) acc += part[((int64_t)q * g.m + j) * g.nrhs + rc];
+  double* yp = g.y + (int64_t)j * g.ldy + rc;
+  *yp = (g.beta != 0.0) ? fma(g.beta, *yp, acc) : acc;
 }
 
 // ------------------------------------------------------------------- plans
@@ -545,10 +598,21 @@ void DevPlan::launch(int idx, hipStream_t st) const {
                        d_prefix + e.pre_off, e.njobs);
 }
 
-void gemv_launch(const GemvJob& g, int form_t, hipStream_t st) {
+size_t gemv_scratch(const GemvJob& g, int form_t) {
+  return form_t ? (size_t)gemv_t_chunks(g) * g.m * g.nrhs : 0;
+}
+
+void gemv_launch(const GemvJob& g, int form_t, double* scratch, hipStream_t st) {
   if (g.m <= 0) return;
-  const int grid = form_t ? (g.m + 15) / 16 : (g.m + 3) / 4;
-  hipLaunchKernelGGL(k_gemv, dim3(grid), dim3(BTA_THREADS), 0, st, g, form_t);
+  if (!form_t) {
+    hipLaunchKernelGGL(k_gemv_n, dim3((g.m + 3) / 4), dim3(BTA_THREADS), 0, st, g);
+    return;
+  }
+  const int nchunk = gemv_t_chunks(g);
+  hipLaunchKernelGGL(k_gemv_t_part, dim3((g.m + 63) / 64, nchunk), dim3(BTA_THREADS), 0, st, g, scratch);
+  const int64_t tot = (int64_t)g.m * g.nrhs;
+  hipLaunchKernelGGL(k_gemv_t_reduce, dim3((unsigned)((tot + BTA_THREADS - 1) / BTA_THREADS)), dim3(BTA_THREADS), 0,
+                     st, g, scratch, nchunk);
 }
 
 }  // namespace bta

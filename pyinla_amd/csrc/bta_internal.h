@@ -20,7 +20,7 @@ enum KernelClass : int {
   KC_PANEL = 0,     // k_panel_step (tile-column factorization)
   KC_UPDATE = 1,    // k_gemm_nt_sub (block updates D_{i+1}, B_{i+1}, T)
   KC_GROUPED = 2,   // k_gemm_grouped (block inverses, selected inversion)
-  KC_GEMV = 3,      // k_gemv (triangular sweeps)
+  KC_GEMV = 3,      // GEMV kernels (triangular sweeps)
   KC_OTHER = 4,
   KC_COUNT = 5
 };

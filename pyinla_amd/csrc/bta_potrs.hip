@@ -9,6 +9,8 @@
 //                    x_i = W_i^T (y_i - L_i^T x_{i+1} - B_i^T x_a)
 // The right-hand side is overwritten in place (N x nrhs, row-major, leading
 // dimension ldrhs), as the reference's in-place pobtas does.
+#include <algorithm>
+
 #include "bta_internal.h"
 
 namespace bta {
@@ -52,6 +54,23 @@ static GemvJob gjob(double* y, int m, int ldy, int nrhs, double beta) {
   return g;
 }
 
+// op(A) = A^T products through the workspace scratch: right-hand sides are
+// taken in groups small enough for the partial sums of the two-pass kernel.
+static void gemv_t(Workspace& ws, GemvJob g, hipStream_t st) {
+  GemvJob one = g;
+  one.nrhs = 1;
+  const size_t per = std::max<size_t>(1, gemv_scratch(one, 1));
+  const int group = (int)std::max<size_t>(1, std::min<size_t>((size_t)g.nrhs, ws.work_doubles / per));
+  for (int c0 = 0; c0 < g.nrhs; c0 += group) {
+    GemvJob h = g;
+    h.nrhs = std::min(group, g.nrhs - c0);
+    h.y = g.y + c0;
+    for (int s = 0; s < h.nseg; ++s) h.seg[s].x = g.seg[s].x + c0;
+    TimedScope ts_(ws, KC_GEMV, st);
+    gemv_launch(h, 1, ws.work, st);
+  }
+}
+
 int potrs_enqueue(Workspace& ws, const double* D, const double* L, const double* B, const double* T,
                   int n, int b, int a, double* rhs, int nrhs, int ldrhs, int trans, hipStream_t st) {
   (void)D;
@@ -65,30 +84,39 @@ int potrs_enqueue(Workspace& ws, const double* D, const double* L, const double*
         const double* xp = rhs + (int64_t)(i - 1) * b * ldrhs;
         GemvJob g = gjob(xi, b, ldrhs, nrhs, 1.0);
         g.seg[g.nseg++] = gseg(L + (int64_t)(i - 1) * b * b, b, 0, xp, ldrhs, b, -1.0);
-        { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(g, 0, st); }
+        { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(g, 0, nullptr, st); }
       }
-      GemvJob w = gjob(tmp, b, nrhs, nrhs, 0.0);
-      w.seg[w.nseg++] = gseg(ws.wfull + (int64_t)i * b * b, b, 0, xi, ldrhs, b, 1.0, 1);
-      { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(w, 0, st); }
-      copy_rows(xi, ldrhs, tmp, nrhs, b, nrhs, st);
+      // y_i = W_i x_i through the scratch, in column groups that fit it
+      const int group = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrhs, ws.work_doubles / b));
+      for (int c0 = 0; c0 < nrhs; c0 += group) {
+        const int nc = std::min(group, nrhs - c0);
+        GemvJob w = gjob(tmp, b, nc, nc, 0.0);
+        w.seg[w.nseg++] = gseg(ws.wfull + (int64_t)i * b * b, b, 0, xi + c0, ldrhs, b, 1.0, 1);
+        { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(w, 0, nullptr, st); }
+        copy_rows(xi + c0, ldrhs, tmp, nc, b, nc, st);
+      }
       if (a > 0) {
         GemvJob ga = gjob(xa, a, ldrhs, nrhs, 1.0);
         ga.seg[ga.nseg++] = gseg(B + (int64_t)i * a * b, b, 0, xi, ldrhs, b, -1.0);
-        { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(ga, 0, st); }
+        { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(ga, 0, nullptr, st); }
       }
     }
     if (a > 0) {
-      GemvJob t = gjob(tmp, a, nrhs, nrhs, 0.0);
-      t.seg[t.nseg++] = gseg(ws.tinv, 64, 0, xa, ldrhs, a, 1.0, 1);
-      { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(t, 0, st); }
-      copy_rows(xa, ldrhs, tmp, nrhs, a, nrhs, st);
+      const int group = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrhs, ws.work_doubles / a));
+      for (int c0 = 0; c0 < nrhs; c0 += group) {
+        const int nc = std::min(group, nrhs - c0);
+        GemvJob t = gjob(tmp, a, nc, nc, 0.0);
+        t.seg[t.nseg++] = gseg(ws.tinv, 64, 0, xa + c0, ldrhs, a, 1.0, 1);
+        { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(t, 0, nullptr, st); }
+        copy_rows(xa + c0, ldrhs, tmp, nc, a, nc, st);
+      }
     }
   } else {
+    // the two-pass T-form products may write y over their own x
     if (a > 0) {
-      GemvJob t = gjob(tmp, a, nrhs, nrhs, 0.0);
+      GemvJob t = gjob(xa, a, ldrhs, nrhs, 0.0);
       t.seg[t.nseg++] = gseg(ws.tinv, 64, 1, xa, ldrhs, a, 1.0, 2);
-      { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(t, 1, st); }
-      copy_rows(xa, ldrhs, tmp, nrhs, a, nrhs, st);
+      gemv_t(ws, t, st);
     }
     for (int i = n - 1; i >= 0; --i) {
       double* xi = rhs + (int64_t)i * b * ldrhs;
@@ -96,11 +124,10 @@ int potrs_enqueue(Workspace& ws, const double* D, const double* L, const double*
       if (i < n - 1)
         g.seg[g.nseg++] = gseg(L + (int64_t)i * b * b, b, 1, rhs + (int64_t)(i + 1) * b * ldrhs, ldrhs, b, -1.0);
       if (a > 0) g.seg[g.nseg++] = gseg(B + (int64_t)i * a * b, b, 1, xa, ldrhs, a, -1.0);
-      if (g.nseg > 0) { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(g, 1, st); }
-      GemvJob w = gjob(tmp, b, nrhs, nrhs, 0.0);
+      if (g.nseg > 0) gemv_t(ws, g, st);
+      GemvJob w = gjob(xi, b, ldrhs, nrhs, 0.0);
       w.seg[w.nseg++] = gseg(ws.wfull + (int64_t)i * b * b, b, 1, xi, ldrhs, b, 1.0, 2);
-      { TimedScope ts_(ws, KC_GEMV, st); gemv_launch(w, 1, st); }
-      copy_rows(xi, ldrhs, tmp, nrhs, b, nrhs, st);
+      gemv_t(ws, w, st);
     }
   }
   return (int)hipGetLastError();
