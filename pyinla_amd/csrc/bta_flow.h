@@ -24,9 +24,9 @@
 // tiles on the chain's path are never grouped.
 //
 // The block update that couples block i to the next one is folded into the
-// same launch: as soon as the columns k..k+S-1 of L_i (and B_i) are final,
+// same launch: as soon as a span of columns of L_i (and B_i) is final,
 // helpers apply D_{i+1} -= L_i L_i^T, B_{i+1} -= B_i L_i^T and T -= B_i B_i^T
-// for those S columns to the tiles they own (op FLOW_NEXT, S = FLOW_SPAN).
+// for those columns to the tiles they own (op FLOW_NEXT, flow_next_end).
 // Nothing in the launch waits on these tasks, so they take the lowest
 // priority of their level and fill the time the helpers would otherwise
 // spend waiting for the chain; the next block's launch reads the results.
@@ -50,6 +50,16 @@ enum : int { FLOW_UPD = 0, FLOW_TRSM = 1, FLOW_ZERO = 2, FLOW_NEXT = 3 };
 
 // Columns of L_i per next-block update task.
 #define FLOW_SPAN 4
+
+// A next-block task covers columns [k, flow_next_end(k, nT)) of L_i: spans of
+// FLOW_SPAN columns, except that each of the last FLOW_SPAN columns is a task
+// of its own.  Those columns are solved only at the end of the panel, when
+// every target still waits for them; single-column tasks keep the work left
+// after the chain's last step small.
+__host__ __device__ inline int flow_next_end(int k, int nT) {
+  const int tail = nT - FLOW_SPAN;
+  return k < tail ? (k + FLOW_SPAN < tail ? k + FLOW_SPAN : tail) : k + 1;
+}
 
 // Task word: op (2 bits) | ρ (10) | c (10) | k (10).
 __host__ __device__ inline uint32_t flow_encode(int op, int rho, int c, int k) {

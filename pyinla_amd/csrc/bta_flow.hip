@@ -29,8 +29,8 @@ namespace bta {
 // column's updates stays balanced; the next-block updates go to the same
 // pool.  Cleared upper tiles of D go by row-major index.
 static int flow_near_pool(const FlowShape& s, int helpers) {
-  if (helpers < 4) return 0;
-  return std::min(3 * s.nT, helpers / 4);
+  if (helpers < 8) return 0;
+  return std::min(3 * s.nT, helpers / 8);
 }
 
 static bool flow_is_near(int nT, int rho, int c) { return rho < nT && rho >= c && rho - c <= 2; }
@@ -77,8 +77,8 @@ static void flow_build_levels(const FlowShape& s, int helpers, int delta, FlowSc
   // next-block updates: one task per target tile and span of S columns of
   // L_i / B_i, at the level of the span's last column, lowest priority
   auto push_next = [&](int rho, int c) {
-    for (int k = 0; k < nT; k += FLOW_SPAN) {
-      const int klast = std::min(k + FLOW_SPAN, nT) - 1;
+    for (int k = 0; k < nT; k = flow_next_end(k, nT)) {
+      const int klast = flow_next_end(k, nT) - 1;
       const unsigned long long key = ((unsigned long long)(3 * klast + 2) << 42) | (1023ull << 31) |
                                      ((unsigned long long)c << 21) | ((unsigned long long)rho << 10) |
                                      (unsigned long long)k;
@@ -225,14 +225,21 @@ static bool flow_build_list(const FlowShape& s, int helpers, FlowSchedule& out) 
         xprod[tid] = t;
       }
     }
-  // next-block updates: one task per target and span of FLOW_SPAN columns
+  // next-block updates: one task per target and span of columns.  Each
+  // target goes to the helper with the least work so far (every helper, the
+  // near pool included: these tasks only fill gaps, at the lowest rank).
+  std::vector<double> load(helpers + 1, 0.0);
+  for (size_t t = 0; t < agent.size(); ++t) load[agent[t]] += cost[t];
   auto push_next = [&](int rho, int c) {
     const int ra = rho < nT ? nT + rho : nT + s.nTL;
     const int rb = rho < nT ? nT + c : (rho == nT ? nT + c : nT + s.nTL);
-    const int owner = flow_next_owner(s, rho, c, helpers);
+    int owner = 0;
+    for (int h = 1; h < helpers; ++h)
+      if (load[h] < load[owner]) owner = h;
+    load[owner] += (double)FLOW_COST_NEXTCOL * nT;
     int prev = -1;
-    for (int k = 0; k < nT; k += FLOW_SPAN) {
-      const int kend = std::min(k + FLOW_SPAN, nT);
+    for (int k = 0; k < nT; k = flow_next_end(k, nT)) {
+      const int kend = flow_next_end(k, nT);
       const int t = add(owner, FLOW_COST_NEXTCOL * (kend - k), FLOW_NEXT, flow_encode(FLOW_NEXT, rho, c, k), -1);
       edge(prev, t);
       prev = t;
@@ -632,7 +639,7 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
     // at once, so that their latency overlaps the staging of this step.
     HStep nxt;
     bool has_next = true;
-    if (cur.op == FLOW_NEXT && cur.kk + 1 < min(cur.k + FLOW_SPAN, nT)) nxt = make_step(p, cur.t, cur.kk + 1);
+    if (cur.op == FLOW_NEXT && cur.kk + 1 < flow_next_end(cur.k, nT)) nxt = make_step(p, cur.t, cur.kk + 1);
     else if (cur.t + 1 < t1) nxt = make_step(p, cur.t + 1, -1);
     else has_next = false;
     const bool probe = has_next && nxt.op != FLOW_ZERO;

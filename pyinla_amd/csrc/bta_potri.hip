@@ -158,16 +158,18 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     double* G = hasL ? Gall + (int64_t)i * b * b : nullptr;
     double* Ga = a > 0 ? Gaall + (int64_t)i * a * b : nullptr;
     GemmLaunch l2(ts), l3(ts);
-    if (hasL) {
-      GemmJob s2 = job(Li, b, b, b, 0.0);  // Sigma_{i+1,i}
-      add_seg(s2, seg(Dn, b, 0, G, b, 0, b, -1.0));
-      if (a > 0) add_seg(s2, seg(Bn, b, 1, Ga, b, 0, a, -1.0));
-      l2.add(s2);
-    }
+    // the few Sigma_{n,i} tiles go first: dispatched ahead of the b x b
+    // tiles they start at once instead of trailing the launch as a second wave
     if (a > 0) {
       GemmJob s2 = job(Bi, a, b, b, 0.0);  // Sigma_{n,i}
       if (hasL) add_seg(s2, seg(Bn, b, 0, G, b, 0, b, -1.0));
       add_seg(s2, seg(T, a, 0, Ga, b, 0, a, -1.0));
+      l2.add(s2);
+    }
+    if (hasL) {
+      GemmJob s2 = job(Li, b, b, b, 0.0);  // Sigma_{i+1,i}
+      add_seg(s2, seg(Dn, b, 0, G, b, 0, b, -1.0));
+      if (a > 0) add_seg(s2, seg(Bn, b, 1, Ga, b, 0, a, -1.0));
       l2.add(s2);
     }
     if (hasL || a > 0) {

@@ -28,6 +28,13 @@ UPD, TRSM, ZERO, NEXT = 0, 1, 2, 3
 SPAN = 4
 
 
+def next_end(k, nT):
+    """End of the next-block task starting at column k (the library's rule:
+    spans of SPAN columns, the last SPAN columns one each)."""
+    tail = nT - SPAN
+    return min(k + SPAN, tail) if k < tail else k + 1
+
+
 def schedule(nT, nTL, nTa, helpers, delta):
     lib = _lib.load()
     n = C.c_int64()
@@ -142,7 +149,7 @@ class Sim:
                     tgt, ra, rb = self.Bn[:, c * t:(c + 1) * t], nT + self.nTL, nT + c
                 else:
                     tgt, ra, rb = self.T, nT + self.nTL, nT + self.nTL
-                for kk in range(k, min(k + SPAN, nT)):
+                for kk in range(k, next_end(k, nT)):
                     yield ("x", ra, kk)
                     if rb != ra:
                         yield ("x", rb, kk)
