@@ -139,7 +139,7 @@ static void flow_build_levels(const FlowShape& s, int helpers, int delta, FlowSc
 // all workgroups resident the earliest unfinished task in that order can
 // always run, whatever the real durations: no interleaving can deadlock.
 #ifndef FLOW_COST_CHOL
-#define FLOW_COST_CHOL 63000  // chain: factor + inverse of a diagonal tile, stores
+#define FLOW_COST_CHOL 45000  // chain: factor + inverse of a diagonal tile, stores
 #endif
 #ifndef FLOW_COST_CTRSM
 #define FLOW_COST_CTRSM 8000  // chain: solve of the sub-diagonal tile, stores

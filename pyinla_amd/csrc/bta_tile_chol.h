@@ -1,67 +1,52 @@
 // In-workgroup Cholesky factorisation and inverse of one 64x64 FP64 tile.
 //
-// Right-looking over 16 panels of 4 columns, grouped in four 16-column tile
-// columns; the matrix A (-> L) and E (-> L^-1, started at I) live in LDS.
-// Panel p = [p, p+4):
-//   phase A (all 256 threads, no cross-wave dependence):
-//     every thread factors the 4x4 diagonal block A_pp = l l^T itself and
-//     forms inv(l) (IEEE sqrt / divide, so the pivots are correctly rounded);
-//     the 4 (60 - p) threads of the rows below solve X = A_rp inv(l)^T, one
-//     element each; the other 4 (p + 4) threads replace the panel rows of E
-//     by inv(l) E_p (columns <= p + 3, the only non-zero ones);
+// Right-looking over 8 panels of 8 columns (two per 16-column tile column);
+// the matrix A (-> L) and E (-> L^-1, started at I) live in LDS.  For panel
+// p = [p, p+8) the 8x8 diagonal block has already been factored,
+// A_pp = u diag(d) u^T, by one wave (its unit lower factor u and 1/sqrt(d)
+// are in LDS scratch).  Then
+//   phase A (all 256 threads, 64 units of 4 threads): a unit owns a row r
+//     below the panel and solves X_r = A_rp inv(L_pp)^T, or a column c < p+8
+//     of the panel rows of E and replaces it by inv(L_pp) E_pc (forward
+//     substitution with u, then scaling by 1/sqrt(d)); the units of
+//     c in [p, p+8) also write row c - p of L_pp = u diag(sqrt d);
 //   phase B (after one barrier):
-//     rank-4 updates with v_mfma_f64_16x16x4f64 (K = 4 is the panel width):
-//     the rest of the current 16-column tile column, A -= X X^T, and the
-//     rows of E below the panel, E_r -= X E_p;
-// and after the four panels of a tile column, the tiles to its right receive
-// the rank-16 update in one pass.  Row r of E ends as row r of L^-1 because
-// every panel step applies the row operations of the forward substitution
-// L W = I to it.  Two barriers per panel, 37 in all.  (Measured alternatives
-// -- a balanced job list over the waves, or one barrier per panel with
-// redundant per-lane operands -- compiled to branchier code and ran 1.5-2x
-// slower on MI355X.)
+//     the first panel of a tile column applies its rank-8 update to the
+//     rest of the tile column (A -= X X^T, v_mfma_f64_16x16x4f64, K = 8);
+//     the second applies the rank-16 update of the whole tile column to
+//     the tiles right of it.  The wave that has just updated the next
+//     panel's diagonal block factors it (LDL^T with one reciprocal per
+//     column on the dependent chain); the three other waves meanwhile update
+//     the rows of E below the panel, E_r -= X_r E_p.
+// Row r of E ends as row r of L^-1 because every panel applies the row
+// operations of the forward substitution L W = I to it.  Two barriers per
+// panel.  The log pivots and the first failing pivot are read off the
+// diagonal of L at the end (a non-positive pivot gives NaN there).
+// (Measured alternatives on MI355X, cycles of one tile: 4-wide panels with
+// the 4x4 factorization repeated by every thread 56 k; 8-wide panels with
+// the 8x8 factorization repeated by every thread 48 k; this scheme 45 k.)
 #pragma once
 #include "bta_device.h"
 
-// 1/x and 1/sqrt(x) from the hardware estimates plus two Newton steps (full
-// FP64 accuracy for normal positive x; a non-positive pivot still propagates
-// as a negative / NaN value and is reported by the caller).
-__device__ __forceinline__ double rcp_f64(double x) {
-  double y = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, y, 1.0);
-  y = fma(y, e, y);
-  e = fma(-x, y, 1.0);
-  return fma(y, e, y);
-}
+// 1/sqrt(x) from the hardware estimate plus two Newton steps (full FP64
+// accuracy for normal positive x), and 1/x with one Newton step (a few ulp).
+// A non-positive pivot propagates as a negative / NaN value and is reported
+// by the caller.
 __device__ __forceinline__ double rsq_f64(double x) {
   double y = __builtin_amdgcn_rsq(x);
   const double h = 0.5 * x;
   y = y * fma(-h * y, y, 1.5);
   return y * fma(-h * y, y, 1.5);
 }
+__device__ __forceinline__ double rcp1_f64(double x) {
+  const double y = __builtin_amdgcn_rcp(x);
+  return fma(y, fma(-x, y, 1.0), y);
+}
 
 struct TileCholResult {
   double logsum;  // sum of log(pivot) over the valid part (thread 0 only)
   int fail;       // first non-positive / NaN pivot in the valid part, or -1 (thread 0 only)
 };
-
-// 16x16 output tile at (r0, c0) of LDS matrix C (stride BTA_LDS_LD):
-//   C += alpha * sum_{k in [0,K)} A[ra + i][ka + k] * B[kb + k][cb + j]   (bT == false)
-//   C += alpha * sum_k A[ra + i][ka + k] * B[cb + j][kb + k]              (bT == true)
-// computed by the calling wave with v_mfma_f64_16x16x4f64; K % 4 == 0.
-template <bool BT>
-__device__ __forceinline__ f64x4 mma16(const double* __restrict__ A, int ra, int ka, const double* __restrict__ B,
-                                       int kb, int cb, int K, double alpha, f64x4 acc) {
-  const int lane = threadIdx.x & 63;
-  const int li = lane & 15, lk = lane >> 4;
-#pragma unroll 1
-  for (int k = 0; k < K; k += 4) {
-    const double a = alpha * A[(ra + li) * BTA_LDS_LD + ka + k + lk];
-    const double b = BT ? B[(cb + li) * BTA_LDS_LD + kb + k + lk] : B[(kb + k + lk) * BTA_LDS_LD + cb + li];
-    acc = mfma_f64(a, b, acc);
-  }
-  return acc;
-}
 
 __device__ __forceinline__ f64x4 load16(const double* __restrict__ C, int r0, int c0) {
   const int lane = threadIdx.x & 63;
@@ -101,17 +86,91 @@ __device__ __forceinline__ void store16_lower(double* __restrict__ C, int r0, in
   }
 }
 
+// 16x16 tile C(ra, rb) -= sum_{k<16} A[ra+i][ka+k] A[rb+j][ka+k] (LDS stride
+// BTA_LDS_LD), K = 16 unrolled so that all operand reads are issued first.
+__device__ __forceinline__ f64x4 mma16_k16_nt(const double* __restrict__ A, int ra, int rb, int ka, f64x4 acc) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  double a[4], b[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = A[(ra + li) * BTA_LDS_LD + ka + 4 * k + lk];
+    b[k] = A[(rb + li) * BTA_LDS_LD + ka + 4 * k + lk];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) acc = mfma_f64(-a[k], b[k], acc);
+  return acc;
+}
+
+// Rows of E in the 16x16 tile (rt, ct) below panel p: E_r -= X_r E_p, with X
+// the panel columns [p, p+8) of s below row p+7 and E_p the panel rows of E.
+__device__ __forceinline__ void e_rows_update(const double* __restrict__ s, double* __restrict__ w, int p, int rt,
+                                              int ct) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  const int ra = 16 * rt + li;
+  const double xa0 = (ra > p + 7) ? s[ra * BTA_LDS_LD + p + lk] : 0.0;
+  const double xa1 = (ra > p + 7) ? s[ra * BTA_LDS_LD + p + 4 + lk] : 0.0;
+  const double eb0 = w[(p + lk) * BTA_LDS_LD + 16 * ct + li];
+  const double eb1 = w[(p + 4 + lk) * BTA_LDS_LD + 16 * ct + li];
+  f64x4 e = load16(w, 16 * rt, 16 * ct);
+  e = mfma_f64(-xa0, eb0, e);
+  e = mfma_f64(-xa1, eb1, e);
+  store16_below(w, 16 * rt, 16 * ct, e, p + 7, -1, false);
+}
+
+// LDL^T of the 8x8 block at (p, p) of s (lower part), computed by the calling
+// wave; lane 0 stores the unit lower factor to ub[8 i + j] (i > j), 1/sqrt(d)
+// to gb[0..8) and sqrt(d) to gb[8..16).
+__device__ __forceinline__ void panel_ldlt8(const double* __restrict__ s, int p, double* __restrict__ ub,
+                                            double* __restrict__ gb) {
+  const double* dg = s + p * BTA_LDS_LD + p;
+  double a[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) a[i][j] = dg[i * BTA_LDS_LD + j];
+  double d[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    d[j] = a[j][j];
+    const double qj = rcp1_f64(d[j]);
+    double tc[8];
+#pragma unroll
+    for (int i = j + 1; i < 8; ++i) {
+      tc[i] = a[i][j];
+      a[i][j] = tc[i] * qj;
+    }
+#pragma unroll
+    for (int i = j + 1; i < 8; ++i)
+#pragma unroll
+      for (int c = j + 1; c <= i; ++c) a[i][c] = fma(-a[i][j], tc[c], a[i][c]);
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int i = 1; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < i; ++j) ub[8 * i + j] = a[i][j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const double g = rsq_f64(d[j]);
+      gb[j] = g;
+      gb[8 + j] = d[j] * g;
+    }
+  }
+}
+
 // s: 64x64 LDS tile (stride BTA_LDS_LD) holding the SPD matrix in its lower
 //    triangle (upper triangle ignored).  Rows/cols >= mv are treated as identity.
 // On exit: s holds L (lower, zero strict upper, identity outside mv),
 //          w holds L^-1 (lower), both with stride BTA_LDS_LD.
-// scratch: LDS, >= 64 doubles.
+// scratch: LDS, >= 80 doubles.
 __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, double* __restrict__ w,
                                                          double* __restrict__ scratch, int mv) {
   const int tid = threadIdx.x;
   const int wv = tid >> 6, lane = tid & 63;
-  double* piv = scratch;  // [64] pivots (squared diagonal of L)
-  int fail = -1;
+  double* ub = scratch;       // [8][8] unit lower factor of the panel's diagonal block
+  double* gb = scratch + 64;  // [8] 1 / sqrt(d), then [8] sqrt(d)
 
   // identity outside the valid block, zero strict upper triangle (the
   // updates below never store above the diagonal), E = I
@@ -121,136 +180,113 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
     w[i * BTA_LDS_LD + j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
+  if (wv == 0) panel_ldlt8(s, 0, ub, gb);
+  __syncthreads();
 
 #pragma unroll 1
-  for (int ts = 0; ts < 4; ++ts) {
-#pragma unroll 1
-    for (int s4 = 0; s4 < 4; ++s4) {
-      const int p = 16 * ts + 4 * s4;
-      // ---- phase A.  Each thread first issues the LDS reads of its own
-      // operands (a row of the panel below, or a column of the panel rows of
-      // E), so that their latency overlaps the 4x4 factorization.
-      const int nrow = 60 - p;  // rows below the panel
-      const int q = tid & 3;
-      const bool solve_row = tid < 4 * nrow;
-      double* opnd = solve_row ? s + (p + 4 + (tid >> 2)) * BTA_LDS_LD + p
-                               : w + p * BTA_LDS_LD + ((tid - 4 * nrow) >> 2);
+  for (int k = 0; k < 8; ++k) {
+    const int ts = k >> 1, h = k & 1;
+    const int p = 8 * k;
+    // ---- phase A.  Units 0 .. 55-p: rows p+8+u; the rest: columns
+    // c = u - (56-p) of the panel rows of E.  The operands are read first so
+    // that their latency overlaps the reads of the factor.
+    {
+      const int nrow = 56 - p;
+      const int u = tid >> 2, t = tid & 3;
+      const bool solve_row = u < nrow;
+      double* opnd = solve_row ? s + (p + 8 + u) * BTA_LDS_LD + p : w + p * BTA_LDS_LD + (u - nrow);
       const int ostep = solve_row ? 1 : BTA_LDS_LD;
-      const double x0 = opnd[0], x1 = opnd[ostep], x2 = opnd[2 * ostep], x3 = opnd[3 * ostep];
-      const double* dg = s + p * BTA_LDS_LD + p;
-      const double a00 = dg[0];
-      const double a10 = dg[BTA_LDS_LD], a11 = dg[BTA_LDS_LD + 1];
-      const double a20 = dg[2 * BTA_LDS_LD], a21 = dg[2 * BTA_LDS_LD + 1], a22 = dg[2 * BTA_LDS_LD + 2];
-      const double a30 = dg[3 * BTA_LDS_LD], a31 = dg[3 * BTA_LDS_LD + 1], a32 = dg[3 * BTA_LDS_LD + 2];
-      const double a33 = dg[3 * BTA_LDS_LD + 3];
-      // LDL^T of the 4x4 block: the dependent chain holds one reciprocal per
-      // column; square roots come after, off the chain.  l = u diag(sqrt d).
-      const double d0 = a00, q0 = rcp_f64(d0);
-      const double u10 = a10 * q0, u20 = a20 * q0, u30 = a30 * q0;
-      const double d1 = fma(-u10, a10, a11), q1 = rcp_f64(d1);
-      const double t21 = fma(-u20, a10, a21), t31 = fma(-u30, a10, a31);
-      const double u21 = t21 * q1, u31 = t31 * q1;
-      const double d2 = fma(-u21, t21, fma(-u20, a20, a22)), q2 = rcp_f64(d2);
-      const double t32 = fma(-u31, t21, fma(-u30, a20, a32));
-      const double u32 = t32 * q2;
-      const double d3 = fma(-u32, t32, fma(-u31, t31, fma(-u30, a30, a33)));
-      const double g0 = rsq_f64(d0), g1 = rsq_f64(d1), g2 = rsq_f64(d2), g3 = rsq_f64(d3);
-      const double s0 = d0 * g0, s1 = d1 * g1, s2 = d2 * g2, s3 = d3 * g3;  // sqrt(d)
-      // inv(l) = diag(1/sqrt d) inv(u), inv(u) unit lower
-      const double v10 = -u10, v21 = -u21, v32 = -u32;
-      const double v20 = -fma(u21, v10, u20);
-      const double v31 = -fma(u32, v21, u31);
-      const double v30 = -fma(u32, v20, fma(u31, v10, u30));
-      const double r0 = g0, r1 = g1, r2 = g2, r3 = g3;
-      const double i10 = g1 * v10, i20 = g2 * v20, i21 = g2 * v21;
-      const double i30 = g3 * v30, i31 = g3 * v31, i32 = g3 * v32;
-      const double l00 = s0, l10 = u10 * s0, l20 = u20 * s0, l30 = u30 * s0;
-      const double l11 = s1, l21 = u21 * s1, l31 = u31 * s1;
-      const double l22 = s2, l32 = u32 * s2, l33 = s3;
-      if (tid == 0) {
-        piv[p] = d0;
-        piv[p + 1] = d1;
-        piv[p + 2] = d2;
-        piv[p + 3] = d3;
-        if (fail < 0) {
-          if (p < mv && !(d0 > 0.0)) fail = p;
-          else if (p + 1 < mv && !(d1 > 0.0)) fail = p + 1;
-          else if (p + 2 < mv && !(d2 > 0.0)) fail = p + 2;
-          else if (p + 3 < mv && !(d3 > 0.0)) fail = p + 3;
+      double x[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) x[m] = opnd[m * ostep];
+      double uu[8][8], g[8];
+#pragma unroll
+      for (int i = 1; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < i; ++j) uu[i][j] = ub[8 * i + j];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) g[j] = gb[j];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int i = c + 1; i < 8; ++i) x[i] = fma(-uu[i][c], x[c], x[i]);
+      // thread t stores outputs 2t and 2t+1 of its unit; the 4 threads of a
+      // unit are adjacent lanes of one wave, so all of their reads precede
+      // the stores
+      const double v0 = t == 0 ? x[0] * g[0] : (t == 1 ? x[2] * g[2] : (t == 2 ? x[4] * g[4] : x[6] * g[6]));
+      const double v1 = t == 0 ? x[1] * g[1] : (t == 1 ? x[3] * g[3] : (t == 2 ? x[5] * g[5] : x[7] * g[7]));
+      opnd[(2 * t) * ostep] = v0;
+      opnd[(2 * t + 1) * ostep] = v1;
+      if (u >= 56) {  // column c = p + di of E: also row di of L_pp
+        const int di = u - 56;
+        double* lrow = s + (p + di) * BTA_LDS_LD + p;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int j = 2 * t + e;
+          lrow[j] = j < di ? ub[8 * di + j] * gb[8 + j] : (j == di ? gb[8 + j] : 0.0);
         }
       }
-      // rows below: X[r][q] = sum_{m<=q} A[r][p+m] inv(l)[q][m];
-      // panel rows of E: E[p+q][c] = sum_{m<=q} inv(l)[q][m] E[p+m][c] (c <= p+3).
-      // The 4 threads sharing a row / column are adjacent lanes of one wave,
-      // so all of their reads precede the write.
-      {
-        double v;
-        if (q == 0) v = x0 * r0;
-        else if (q == 1) v = fma(x0, i10, x1 * r1);
-        else if (q == 2) v = fma(x0, i20, fma(x1, i21, x2 * r2));
-        else v = fma(x0, i30, fma(x1, i31, fma(x2, i32, x3 * r3)));
-        opnd[q * ostep] = v;
-      }
-      __syncthreads();
-      // ---- phase B: write l; rank-4 updates of the tile column and of E
-      if (tid < 16) {
-        const int i = tid >> 2, j = tid & 3;
-        double v;
-        if (i == 0) v = (j == 0) ? l00 : 0.0;
-        else if (i == 1) v = (j == 0) ? l10 : (j == 1 ? l11 : 0.0);
-        else if (i == 2) v = (j == 0) ? l20 : (j == 1 ? l21 : (j == 2 ? l22 : 0.0));
-        else v = (j == 0) ? l30 : (j == 1 ? l31 : (j == 2 ? l32 : l33));
-        s[(p + i) * BTA_LDS_LD + p + j] = v;
-      }
-      if (wv >= ts) {
-        const int li = lane & 15, lk = lane >> 4;
-        const int ra = 16 * wv + li;  // row of the A operand (this wave's tile row)
-        const int rb = 16 * ts + li;  // row of the B operand (tile column ts)
-        const double xa = (ra > p + 3) ? s[ra * BTA_LDS_LD + p + lk] : 0.0;
-        const double xb = (rb > p + 3) ? s[rb * BTA_LDS_LD + p + lk] : 0.0;
-        f64x4 acc = mfma_f64(-xa, xb, load16(s, 16 * wv, 16 * ts));
-        // rows of E in this tile row below the panel: E_r -= X_r E_p, tiles (wv, 0..ts)
-        f64x4 eacc[4];
-#pragma unroll
-        for (int ci = 0; ci < 4; ++ci) {
-          if (ci <= ts) {
-            const double eb = w[(p + lk) * BTA_LDS_LD + 16 * ci + li];
-            eacc[ci] = mfma_f64(-xa, eb, load16(w, 16 * wv, 16 * ci));
-          }
-        }
-        store16_below(s, 16 * wv, 16 * ts, acc, p + 3, p + 3, wv == ts);
-#pragma unroll
-        for (int ci = 0; ci < 4; ++ci)
-          if (ci <= ts) store16_below(w, 16 * wv, 16 * ci, eacc[ci], p + 3, -1, false);
-      }
-      __syncthreads();
     }
-    // tiles right of tile column ts: A[ri][ci] -= X[ri] X[ci]^T (K = 16)
-    const int nr = 3 - ts;
-    if (nr > 0) {
+    __syncthreads();
+    // ---- phase B
+    const int li = lane & 15, lk = lane >> 4;
+    const int dw = h == 0 ? ts : 0;  // wave that updates and factors the next diagonal block
+    if (h == 0) {
+      // rank-8 update of the rest of tile column ts
+      if (wv >= ts) {
+        const int ra = 16 * wv + li, rb = 16 * ts + li;
+        const double xa0 = (ra > p + 7) ? s[ra * BTA_LDS_LD + p + lk] : 0.0;
+        const double xa1 = (ra > p + 7) ? s[ra * BTA_LDS_LD + p + 4 + lk] : 0.0;
+        const double xb0 = (rb > p + 7) ? s[rb * BTA_LDS_LD + p + lk] : 0.0;
+        const double xb1 = (rb > p + 7) ? s[rb * BTA_LDS_LD + p + 4 + lk] : 0.0;
+        f64x4 acc = load16(s, 16 * wv, 16 * ts);
+        acc = mfma_f64(-xa0, xb0, acc);
+        acc = mfma_f64(-xa1, xb1, acc);
+        store16_below(s, 16 * wv, 16 * ts, acc, p + 7, p + 7, wv == ts);
+      }
+    } else if (ts < 3) {
+      // tiles right of tile column ts: A[ri][ci] -= X[ri] X[ci]^T (K = 16);
+      // wave 0 takes the diagonal tile (ts+1, ts+1) alone, waves 1-3 the rest
+      const int nr = 3 - ts;
       const int ntr = nr * (nr + 1) / 2;
+      const int t0 = wv, tstep = wv == 0 ? ntr : 3;
 #pragma unroll 1
-      for (int t = wv; t < ntr; t += 4) {
-        int rr = 0, cc = t;
+      for (int tt = t0; tt < ntr; tt += tstep) {
+        int rr = 0, cc = tt;
         while (cc > rr) {
           cc -= rr + 1;
           ++rr;
         }
         const int ri = ts + 1 + rr, ci = ts + 1 + cc;
         f64x4 acc = load16(s, 16 * ri, 16 * ci);
-        acc = mma16<true>(s, 16 * ri, 16 * ts, s, 16 * ts, 16 * ci, 16, -1.0, acc);
+        acc = mma16_k16_nt(s, 16 * ri, 16 * ci, 16 * ts, acc);
         if (ri == ci) store16_lower(s, 16 * ri, 16 * ci, acc);
         else store16(s, 16 * ri, 16 * ci, acc);
       }
-      __syncthreads();
     }
+    if (k < 7 && wv == dw) {
+      panel_ldlt8(s, p + 8, ub, gb);  // reads back this wave's own stores
+    } else {
+      // rows of E below the panel, over the three other waves
+      const int rt0 = h == 0 ? ts : ts + 1;
+      const int ncol = ts + 1;
+      const int njobs = (4 - rt0) * ncol;
+      const int slot = wv < dw ? wv : wv - 1;
+#pragma unroll 1
+      for (int j = slot; j < njobs; j += 3) e_rows_update(s, w, p, rt0 + j / ncol, j % ncol);
+    }
+    __syncthreads();
   }
-  // sum of log pivots: fixed butterfly over wave 0 (deterministic order)
+  // sum of log pivots (log d_i = 2 log l_ii): fixed butterfly over wave 0
+  // (deterministic order); first non-positive pivot from the diagonal of L
   TileCholResult res;
-  res.fail = fail;
+  res.fail = -1;
   res.logsum = 0.0;
   if (wv == 0) {
-    double v = (lane < mv) ? log(piv[lane]) : 0.0;
+    const double lii = s[lane * BTA_LDS_LD + lane];
+    const unsigned long long bad = __ballot(lane < mv && !(lii > 0.0));
+    res.fail = bad ? __ffsll((long long)bad) - 1 : -1;
+    double v = (lane < mv) ? 2.0 * log(lii) : 0.0;
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
     res.logsum = v;
