@@ -159,6 +159,39 @@ __device__ __forceinline__ void mma_nt(TileAcc& t, const double* __restrict__ sA
   }
 }
 
+// Workgroup barrier ordering LDS accesses only.  __syncthreads() also waits
+// for every outstanding global load of the wave (its release fence covers all
+// address spaces), which would stall a wave at the barrier until an operand
+// prefetch meant for the next step has arrived.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// mma_nt over the full K = 64 of two LDS tiles, unrolled: the operand reads of
+// later k steps are issued while earlier MFMAs run.
+template <bool NEG>
+__device__ __forceinline__ void mma_nt64(TileAcc& t, const double* __restrict__ sA,
+                                         const double* __restrict__ sB) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* a0p = sA + (wm + li) * BTA_LDS_LD + lk;
+  const double* a1p = sA + (wm + 16 + li) * BTA_LDS_LD + lk;
+  const double* b0p = sB + (wn + li) * BTA_LDS_LD + lk;
+  const double* b1p = sB + (wn + 16 + li) * BTA_LDS_LD + lk;
+#pragma unroll
+  for (int k = 0; k < 64; k += 4) {
+    double a0 = a0p[k], a1 = a1p[k], b0 = b0p[k], b1 = b1p[k];
+    if (NEG) { a0 = -a0; a1 = -a1; }
+    t.c[0][0] = mfma_f64(a0, b0, t.c[0][0]);
+    t.c[0][1] = mfma_f64(a0, b1, t.c[0][1]);
+    t.c[1][0] = mfma_f64(a1, b0, t.c[1][0]);
+    t.c[1][1] = mfma_f64(a1, b1, t.c[1][1]);
+  }
+}
+
 // Load / store the accumulator from / to a row-major global block (masked).
 // Addresses are formed as one per-lane base pointer plus wave-uniform offsets
 // (16x + 4r) * ld + 16y, so that only the base occupies vector registers.

@@ -681,20 +681,20 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
     if (cur.srcB != cur.srcA) pb.store(sB, 0);
     // the following step's operands are fetched now if they are ready
     if (threadIdx.x == 0) *ok_sh = (probe && fv1 == tag && fv2 == tag) ? 1 : 0;
-    __syncthreads();
+    lds_barrier();
     fetched = (*ok_sh != 0);
     if (fetched) step_fetch(nxt, pa, pb);
     const double* sBop = (cur.srcB == cur.srcA) ? sA : sB;
     if (cur.op == FLOW_TRSM) {
       flow_trsm(acc, sA, sB);  // the accumulator was staged in sA: reuse its registers
-      __syncthreads();
+      lds_barrier();
       acc_store(acc, g, p.b, rows, cols);
       acc_store_sys(acc, xtile(p, cur.rho, cur.c));
       carry_rho = carry_c = -1;
       flow_publish(xflag(p, cur.rho, cur.c), tag);
     } else {
-      mma_nt<true>(acc, sA, sBop, 0, 64);
-      __syncthreads();
+      mma_nt64<true>(acc, sA, sBop);
+      lds_barrier();
       if (cur.op == FLOW_NEXT) {
         if (!(has_next && nxt.op == FLOW_NEXT && nxt.t == cur.t)) acc_store(acc, tg, ld, trows, tcols);
         carry_rho = carry_c = -1;

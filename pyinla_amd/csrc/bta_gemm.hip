@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
     else
 #endif
       mma_nt_alpha<BTA_GEMM_LD>(acc, sA, sB, (kw + 3) & ~3, alpha);
-    __syncthreads();
+    lds_barrier();  // not a wait for the next chunk's loads: its stores wait for them
   }
   // epilogue
 #pragma unroll
