@@ -162,6 +162,12 @@ static void flow_build_levels(const FlowShape& s, int helpers, int delta, FlowSc
 #ifndef FLOW_COST_ZERO
 #define FLOW_COST_ZERO 500
 #endif
+#ifndef FLOW_UPD_GROUP
+#define FLOW_UPD_GROUP 2  // columns per held-back update group
+#endif
+#ifndef FLOW_UPD_MARGIN
+#define FLOW_UPD_MARGIN 2  // the updates by the last columns before a tile's own are not held back
+#endif
 #ifndef FLOW_COST_LAT
 #define FLOW_COST_LAT 1500  // flag store to flag seen by another workgroup
 #endif
@@ -255,6 +261,17 @@ static bool flow_build_list(const FlowShape& s, int helpers, FlowSchedule& out) 
     flow_decode(word[u.first], o, rho, c, j);
     edge(xprod[(size_t)rho * nT + j], u.first);
     if (rho != c) edge(xprod[(size_t)c * nT + j], u.first);
+    // Updates of a tile by columns far from its own are held back to the
+    // last column of their group of FLOW_UPD_GROUP: the helper then applies
+    // the group back to back with the accumulator kept in registers (one
+    // load and one store of the tile per group instead of per column).  Only
+    // the simulated order sees these edges; the device waits for the real
+    // flags.
+    const int g1 = std::min((j / FLOW_UPD_GROUP + 1) * FLOW_UPD_GROUP - 1, c - 1 - FLOW_UPD_MARGIN);
+    if (g1 > j && !flow_is_handoff(nT, rho, c)) {
+      edge(xprod[(size_t)rho * nT + g1], u.first);
+      if (rho != c) edge(xprod[(size_t)c * nT + g1], u.first);
+    }
   }
   if (s.nTL > 0) {
     for (int r = 0; r < nT; ++r)
