@@ -157,6 +157,13 @@ int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
 int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
               void* stream);
 
+/* Block inverses W_i = D_i^-1 of the diagonal factor blocks, enqueued on
+ * `stream` and kept in the workspace until the next factorization: the
+ * first bta_potrs / bta_potri after a factorization builds them, this lets a
+ * caller build them earlier (e.g. while another solver's work leaves the
+ * device idle).  No counterpart in the reference. */
+int bta_block_inverses(bta_ws_t ws, double* D, int n, int b, void* stream);
+
 /* ---------------------------------------------------------- assembly */
 
 /* Dense BTA blocks of Q = sum_{v,w,m,k} coef[((v*nv+w)*nM+m)*nK+k] *

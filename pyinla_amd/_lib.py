@@ -33,6 +33,7 @@ SIGNATURES = {
     "bta_ws_get_timing": (_I, [_P, _I, C.POINTER(_D), C.POINTER(C.c_int64)]),
     "bta_potrs": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P, _I, _I, _I, _P]),
     "bta_potri": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P]),
+    "bta_block_inverses": (_I, [_P, _P, _I, _I, _P]),
     "bta_assemble_kron": (_I, [_P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _I, _I, _P, _P, _I, _P, _D, _P]),
     "bta_scatter_coo": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _D, _I, _P]),
     "bta_gather_coo": (_I, [_P, _P, _P, _P, _P, _P, C.c_int64, _P]),

@@ -207,6 +207,13 @@ int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
   });
 }
 
+int bta_block_inverses(bta_ws_t ws, double* D, int n, int b, void* stream) {
+  int rc = check_shape(ws, n, b, 0);
+  if (rc) return rc;
+  if (!D) return BTA_E_ARG;
+  return ensure_inverses(ws, D, n, b, as_stream(stream));
+}
+
 int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a, void* stream) {
   int rc = check_shape(ws, n, b, a);
   if (rc) return rc;

@@ -215,6 +215,16 @@ class SerinvSolver:
             return host
         return rhs
 
+    def prepare_block_inverses(self, sparsity: str) -> None:
+        """Enqueue the block inverses of the current factor that the next
+        solve / selected inversion would otherwise build first (no reference
+        counterpart)."""
+        _check_sparsity(sparsity)
+        D = self._blocks(sparsity)[0]
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.bta_block_inverses(self._ws, _ptr(D), self.n_diag_blocks, self.diagonal_blocksize,
+                                                   _stream_ptr(self.device)), "bta_block_inverses")
+
     def selected_inversion(self, sparsity: str) -> None:
         """Overwrite the factor with the selected inverse (structured_solver.py:168-190).
 

@@ -255,14 +255,18 @@ class STGaussianModel:
         cur.wait_stream(self.s_cond)
         cur.wait_stream(self.s_prior)
 
-    def forward(self, theta) -> dict:
+    def forward(self, theta, prior_inverses: bool = False) -> dict:
         """Assemble, factor both matrices (on two streams), solve for x*;
-        returns the pieces of f."""
+        returns the pieces of f.  ``prior_inverses`` also builds the prior's
+        block inverses (needed by its selected inversion) right behind its
+        factorization, where they overlap the conditional solve."""
         theta = np.asarray(theta, dtype=np.float64)
         cur = self._fork()
         with torch.cuda.stream(self.s_prior):
             self.assemble_prior(theta)
             self.prior.factorize("bt")
+            if prior_inverses:
+                self.prior.prepare_block_inverses("bt")
         with torch.cuda.stream(self.s_cond):
             self.assemble_cond(theta)
             self.cond.factorize("bta")
@@ -379,7 +383,7 @@ class STGaussianModel:
     def value_and_grad(self, theta):
         """f(theta) and its adjoint gradient (one forward + selected inversions)."""
         theta = np.asarray(theta, dtype=np.float64)
-        parts = self.forward(theta)
+        parts = self.forward(theta, prior_inverses=True)
         # terms that need x*, taken before the factors are overwritten
         Zq = self._contract(None, None, self.x, 1)
         xAtAx = [self._bilinear(o, 1, x=self.x) for o in self.obs]

@@ -232,3 +232,32 @@ def test_solve_many_rhs(cuda, n, b, a, nrhs):
     x = torch.from_numpy(R.copy()).to(cuda)
     s.solve(x, "bta")
     assert _rel(x.cpu().numpy(), xr) < RTOL
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_block_inverses_built_ahead(cuda, use_graph):
+    """bta_block_inverses before the solve / selected inversion gives the same
+    results as letting them build the inverses, and a refactorization
+    invalidates the prepared inverses."""
+    n, b, a = 3, 200, 5
+    kf = known_factor_bta(n, b, a, seed=321)
+    s = SerinvSolver(None, b, n, a, use_graph=use_graph)
+    rng = np.random.default_rng(9)
+    R = rng.normal(size=(n * b + a, 2))
+    xr = R.copy()
+    ob.pobtas(kf.Lf_D, kf.Lf_L, kf.Lf_B, kf.Lf_T, xr, "N")
+    ob.pobtas(kf.Lf_D, kf.Lf_L, kf.Lf_B, kf.Lf_T, xr, "C")
+    for rep in range(2):
+        # second round: new values (scaled matrix) so that stale inverses would show
+        scale = 1.0 + rep
+        s.load_blocks(kf.Q_D * scale, kf.Q_L * scale, kf.Q_B * scale, kf.Q_T * scale)
+        s.factorize("bta")
+        s.prepare_block_inverses("bta")
+        x = torch.from_numpy(R.copy()).to(cuda)
+        s.solve(x, "bta")
+        assert _rel(x.cpu().numpy(), xr / scale) < RTOL
+    s.selected_inversion("bta")
+    D, L, B, T = kf.Lf_D.copy(), kf.Lf_L.copy(), kf.Lf_B.copy(), kf.Lf_T.copy()
+    ob.pobtasi(D, L, B, T)
+    assert _rel(s.A_diagonal_blocks.cpu().numpy(), D / 2.0) < RTOL
+    assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L / 2.0) < RTOL
