@@ -384,15 +384,18 @@ class STGaussianModel:
         """f(theta) and its adjoint gradient (one forward + selected inversions)."""
         theta = np.asarray(theta, dtype=np.float64)
         parts = self.forward(theta, prior_inverses=True)
-        # terms that need x*, taken before the factors are overwritten
-        Zq = self._contract(None, None, self.x, 1)
-        xAtAx = [self._bilinear(o, 1, x=self.x) for o in self.obs]
-        xAty = [float(torch.dot(self.x, o.Aty)) for o in self.obs]
+        # the prior's selected inversion does not need x*: queued before the
+        # host waits for the x* terms below
         cur = self._fork()
         with torch.cuda.stream(self.s_prior):
             self.prior.selected_inversion("bt")
             op = self._contract_launch(self.prior.A_diagonal_blocks, self.prior.A_lower_diagonal_blocks, None, 0,
                                        work=self.work_p, out=self.contract_out_p)
+        # terms that need x*, taken before the conditional factor is overwritten
+        Zq = self._contract(None, None, self.x, 1)
+        xAtAx = [self._bilinear(o, 1, x=self.x) for o in self.obs]
+        xAty = [float(torch.dot(self.x, o.Aty)) for o in self.obs]
+        self.s_cond.wait_stream(cur)
         with torch.cuda.stream(self.s_cond):
             self.cond.selected_inversion("bta")
             oc = self._contract_launch(self.cond.A_diagonal_blocks, self.cond.A_lower_diagonal_blocks, None, 0)
