@@ -207,6 +207,17 @@ int bta_gather_coo(const double* D, const double* L, const double* B, const doub
  * work[nwork].  No reference counterpart: it evaluates tr(Sigma A^T A) and
  * x^T A^T A x for the adjoint gradient of f(theta), whose finite-difference
  * form is PyINLA._objective_function (pyinla.py:512-600). */
+/* out[j] = a_j^T S a_j for every row a_j of the CSR matrix (rowptr[m+1],
+ * colidx, val; global column indices into the n*b + a unknowns), where S is
+ * the symmetric matrix held on the BTA pattern by D (full symmetric blocks),
+ * L, B and T.  After bta_potri this is diag(A Sigma A^T), the marginal
+ * variances of the observations that PyINLA.get_marginal_variances_observations
+ * forms as (a @ Sigma_sparse @ a.T).diagonal() (pyinla.py:960-1016).  A pair of
+ * columns outside the BTA pattern yields NaN. */
+int bta_quadform_rows(const double* D, const double* L, const double* B, const double* T, int n, int b, int a,
+                      const int* rowptr, const int* colidx, const double* val, int64_t m, double* out,
+                      void* stream);
+
 int bta_bilinear_coo(const double* D, const double* L, const double* B, const double* T,
                      const uint64_t* code, const int* grow, const int* gcol, const double* x,
                      const double* wval, int64_t count, int mode, double* work, int nwork,

@@ -113,6 +113,29 @@ def evaluate_f(prob, table_fn, theta, return_parts=False):
     return f
 
 
+def marginal_variances(prob, table_fn, theta):
+    """Marginal variances of the latent parameters and of the observations
+    (PyINLA.get_marginal_variances_latent_parameters /
+    get_marginal_variances_observations, pyinla.py:912-1016, Gaussian branch):
+    Q_cond(theta) factored and inverted on its BTA pattern (pobtaf, pobtasi);
+    diag(Sigma), then diag(A Sigma_pattern A^T) with Sigma kept on the pattern
+    of Q_cond (as _structured_to_spmatrix(Q_conditional) does,
+    structured_solver.py:470-538).  Returns (latent, [per observation set])."""
+    theta = np.asarray(theta, dtype=np.float64)
+    n, b, a = prob.nt, getattr(prob, "nv", 1) * prob.ns, prob.nb
+    Qc = q_cond(prob, q_prior(prob, table_fn(theta)), theta)
+    D, L, B, T = scatter_bta(Qc, n, b, a, "bta")
+    ob.pobtaf(D, L, B, T)
+    ob.pobtasi(D, L, B, T)
+    latent = np.concatenate([np.concatenate([np.diag(D[i]) for i in range(n)]), np.diag(T)])
+    S = sps.csr_matrix(ob.bta_to_dense(D, L, B, T))
+    pat = sps.csr_matrix(Qc)
+    pat.data[:] = 1.0
+    S = S.multiply(pat).tocsr()  # Sigma on the pattern of Q_cond
+    obs = [np.asarray((A @ S @ A.T).diagonal()).ravel() for A, _, _ in _obs(prob)]
+    return latent, obs
+
+
 def fd_gradient(prob, table_fn, theta, eps=1e-3):
     """Central FD with the reference's evaluation order (pyinla.py:540-580)."""
     theta = np.asarray(theta, dtype=np.float64)

@@ -38,6 +38,7 @@ SIGNATURES = {
     "bta_scatter_coo": (_I, [_P, _P, _P, _P, _P, _P, _P, C.c_int64, _D, _I, _P]),
     "bta_gather_coo": (_I, [_P, _P, _P, _P, _P, _P, C.c_int64, _P]),
     "bta_bilinear_coo": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int64, _I, _P, _I, _P, _P]),
+    "bta_quadform_rows": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, C.c_int64, _P, _P]),
     "bta_kron_contract": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P]),
     "bta_flow_schedule": (_I, [_I, _I, _I, _I, _I, _P, C.c_int64, _P, C.POINTER(C.c_int64)]),
 }

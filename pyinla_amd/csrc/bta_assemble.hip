@@ -158,6 +158,51 @@ __global__ void __launch_bounds__(BTA_THREADS) k_bilinear_partial(const double* 
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// Entry (k, l) of a symmetric matrix held on the BTA pattern (D_i full
+// symmetric blocks, L_i the block below D_i, B_i the arrow row, T the tip);
+// NaN for a pair outside the pattern.
+__device__ __forceinline__ double bta_sym_at(const double* __restrict__ D, const double* __restrict__ L,
+                                             const double* __restrict__ B, const double* __restrict__ T,
+                                             int b, int a, int64_t N, int64_t k, int64_t l) {
+  if (k < l) {
+    const int64_t t = k;
+    k = l;
+    l = t;
+  }
+  // now k >= l
+  if (k >= N) {
+    const int64_t r = k - N;
+    if (l >= N) return T[r * a + (l - N)];
+    const int64_t bl = l / b;
+    return B[(bl * a + r) * b + (l - bl * b)];
+  }
+  const int64_t bk = k / b, bl = l / b;
+  const int64_t ik = k - bk * b, il = l - bl * b;
+  if (bk == bl) return D[(bk * b + ik) * b + il];
+  if (bk == bl + 1) return L[(bl * b + ik) * b + il];
+  return __builtin_nan("");
+}
+
+// out[j] = a_j^T S a_j for the rows a_j of a CSR matrix (one thread per row),
+// S on the BTA pattern: the marginal variances diag(A Sigma A^T) of the
+// observations once S holds the selected inverse.
+__global__ void __launch_bounds__(BTA_THREADS) k_quadform_rows(const double* D, const double* L,
+                                                               const double* B, const double* T, int b, int a,
+                                                               int64_t N, const int* rowptr, const int* colidx,
+                                                               const double* val, int64_t m, double* out) {
+  for (int64_t j = (int64_t)blockIdx.x * BTA_THREADS + threadIdx.x; j < m; j += (int64_t)gridDim.x * BTA_THREADS) {
+    const int p0 = rowptr[j], p1 = rowptr[j + 1];
+    double acc = 0.0;
+    for (int p = p0; p < p1; ++p) {
+      const int64_t kp = colidx[p];
+      const double vp = val[p];
+      acc = fma(vp * vp, bta_sym_at(D, L, B, T, b, a, N, kp, kp), acc);
+      for (int q = p + 1; q < p1; ++q) acc = fma(2.0 * vp * val[q], bta_sym_at(D, L, B, T, b, a, N, kp, colidx[q]), acc);
+    }
+    out[j] = acc;
+  }
+}
+
 // Kronecker contractions, per block t, field pair (v, w) and spatial matrix k:
 //   mode 0 (traces):     PD = <D_t[v,w], S_k>_F,  PL = <L_t[v,w], S_k>_F
 //   mode 1 (quadratic):  PD = x_{t,v}^T S_k x_{t,w},  PL = x_{t+1,v}^T S_k x_{t,w}
@@ -321,6 +366,18 @@ int bta_gather_coo(const double* D, const double* L, const double* B, const doub
   const int grid = (int)std::min<int64_t>((count + BTA_THREADS - 1) / BTA_THREADS, 4096);
   hipLaunchKernelGGL(bta::k_gather, dim3(grid), dim3(BTA_THREADS), 0, reinterpret_cast<hipStream_t>(stream), D, L,
                      B, T, reinterpret_cast<const unsigned long long*>(code), out, count);
+  return (int)hipGetLastError();
+}
+
+int bta_quadform_rows(const double* D, const double* L, const double* B, const double* T, int n, int b, int a,
+                      const int* rowptr, const int* colidx, const double* val, int64_t m, double* out,
+                      void* stream) {
+  if (n < 1 || b < 1 || a < 0 || m < 0 || !D || (n > 1 && !L) || (a > 0 && (!B || !T))) return BTA_E_ARG;
+  if (m == 0) return 0;
+  if (!rowptr || !colidx || !val || !out) return BTA_E_ARG;
+  const int grid = (int)std::min<int64_t>((m + BTA_THREADS - 1) / BTA_THREADS, 8192);
+  hipLaunchKernelGGL(bta::k_quadform_rows, dim3(grid), dim3(BTA_THREADS), 0, reinterpret_cast<hipStream_t>(stream),
+                     D, L, B, T, b, a, (int64_t)n * b, rowptr, colidx, val, m, out);
   return (int)hipGetLastError();
 }
 

@@ -94,6 +94,17 @@ class _ObsSet:
         self.Aty = torch.from_numpy(np.asarray(A.T @ y, dtype=np.float64)).to(dev)
         self.yty = float(np.asarray(y) @ np.asarray(y))
         self.m = A.shape[0]
+        self._A = A  # host CSR, uploaded on first use by device_csr()
+        self._A_dev = None
+
+    def device_csr(self, dev):
+        """(rowptr, colidx, val) of A on the device (int32, int32, float64)."""
+        if self._A_dev is None:
+            A = self._A
+            self._A_dev = (torch.from_numpy(A.indptr.astype(np.int32)).to(dev),
+                           torch.from_numpy(A.indices.astype(np.int32)).to(dev),
+                           torch.from_numpy(A.data.astype(np.float64)).to(dev))
+        return self._A_dev
 
 
 class STGaussianModel:
@@ -413,6 +424,43 @@ class STGaussianModel:
             g[o.idx] += 0.5 * eo * tr - eo * xb + 0.5 * eo * q - 0.5 * o.m + 0.5 * eo * o.yty
         g -= gaussian_log_prior_grad(theta, self.prob.prior_mean, self.prob.prior_prec)
         return parts["f"], g
+
+
+    def marginal_variances(self, theta, observations: bool = True):
+        """Marginal variances of the latent parameters and of the
+        observations at theta, as PyINLA.get_marginal_variances_latent_parameters
+        and get_marginal_variances_observations compute them for a Gaussian
+        likelihood (pyinla.py:912-1016): Q_cond(theta) is factored and inverted
+        on its BTA pattern; diag(Sigma) is read off the diagonal blocks (latent
+        order of the solver: time blocks, then the fixed effects), and
+        diag(A Sigma A^T) is formed per observation row on the device
+        (bta_quadform_rows) instead of the reference's sparse triple product.
+        Returns (latent variances, [observation variances per observation
+        set]) as device tensors.  Overwrites the conditional solver's factor."""
+        theta = np.asarray(theta, dtype=np.float64)
+        s = self.cond
+        self.assemble_cond(theta)
+        s.factorize("bta")
+        info, _ = s.status()
+        if info:
+            raise np.linalg.LinAlgError(f"not positive definite (cond info {info})")
+        s.selected_inversion("bta")
+        n, b, a = self.n, self.b, self.a
+        parts = [torch.diagonal(s.A_diagonal_blocks, dim1=1, dim2=2).reshape(-1)]
+        if a:
+            parts.append(torch.diagonal(s.A_arrow_tip_block))
+        latent = torch.cat(parts)
+        obs = []
+        if observations:
+            for o in self.obs:
+                rp, ci, v = o.device_csr(self.device)
+                out = torch.empty(o.m, dtype=torch.float64, device=self.device)
+                _lib.check(self.lib.bta_quadform_rows(
+                    _ptr(s.A_diagonal_blocks), _ptr(s.A_lower_diagonal_blocks), _ptr(s.A_arrow_bottom_blocks),
+                    _ptr(s.A_arrow_tip_block), n, b, a, _ptr(rp), _ptr(ci), _ptr(v), o.m, _ptr(out),
+                    _stream_ptr(self.device)), "bta_quadform_rows")
+                obs.append(out)
+        return latent, obs
 
 
 class AdjointF(torch.autograd.Function):

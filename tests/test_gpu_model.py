@@ -184,3 +184,27 @@ def test_lbfgs_adjoint_matches_fd_run(cuda, small):
     r_fd = minimize(model, small.theta, gradient="fd", max_iter=8)
     assert r_adj["f"] <= model.f(small.theta)
     assert np.abs(r_adj["theta"] - r_fd["theta"]).max() < 5e-3
+
+
+@pytest.mark.parametrize("kind", ["st", "coregional"])
+def test_marginal_variances_match_oracle(cuda, small, kind):
+    """diag(Sigma) and diag(A Sigma A^T) (bta_quadform_rows) against the
+    oracle's selected inversion (pinned to the dense inverse on the CPU), then
+    f is unaffected by the overwritten factor (the next evaluation refactors)."""
+    if kind == "st":
+        prob, cm = small, CoefficientModel("baseline")
+        model = STGaussianModel(prob, device=cuda)
+    else:
+        from pyinla_amd.coefficients import CoregionalCoefficients
+        from pyinla_amd.workloads import coregional_problem
+        prob = coregional_problem(nv=2, nt=4, nx=6, ny=5, nb_per=2, obs_per_node=2, seed=8)
+        cm = CoregionalCoefficients(2)
+        model = STGaussianModel(prob, coef=cm, device=cuda)
+    lat, obs = model.marginal_variances(prob.theta)
+    lat_ref, obs_ref = om.marginal_variances(prob, cm.table, prob.theta)
+    assert _rel(lat.cpu().numpy(), lat_ref) < 1e-9
+    assert len(obs) == len(obs_ref)
+    for v, r in zip(obs, obs_ref):
+        assert _rel(v.cpu().numpy(), r) < 1e-9
+    ref = om.evaluate_f(prob, cm.table, prob.theta, return_parts=True)
+    assert abs(model.f(prob.theta) - ref["f"]) <= 1e-10 * max(abs(ref["ld_c"]), abs(ref["f"]))

@@ -98,3 +98,23 @@ def test_workload_generators_are_seeded():
     a = st_problem(nt=3, nx=4, ny=4, nb=2, seed=9)
     b = st_problem(nt=3, nx=4, ny=4, nb=2, seed=9)
     assert (a.A != b.A).nnz == 0 and np.array_equal(a.y, b.y) and np.array_equal(a.theta, b.theta)
+
+
+@pytest.mark.parametrize("kind", ["st", "coregional"])
+def test_marginal_variances_against_dense_inverse(kind):
+    """diag(Sigma) and diag(A Sigma A^T) from the BTA selected inversion equal
+    the same quantities from the dense inverse of Q_cond (every pair of
+    columns of an observation row lies on the pattern of A^T A)."""
+    if kind == "st":
+        p = st_problem(nt=5, nx=4, ny=4, nb=3, seed=4)
+        cm = CoefficientModel("baseline")
+    else:
+        p = coregional_problem(nv=2, nt=3, nx=4, ny=3, nb_per=2, obs_per_node=2, seed=3)
+        cm = CoregionalCoefficients(2)
+    lat, obs = om.marginal_variances(p, cm.table, p.theta)
+    Qc = om.q_cond(p, om.q_prior(p, cm.table(np.asarray(p.theta))), np.asarray(p.theta))
+    S = np.linalg.inv(Qc.toarray())
+    assert np.abs(lat - np.diag(S)).max() <= 1e-10 * np.abs(np.diag(S)).max()
+    for (A, _, _), v in zip(om._obs(p), obs):
+        ref = np.einsum("ij,jk,ik->i", A.toarray(), S, A.toarray())
+        assert np.abs(v - ref).max() <= 1e-10 * np.abs(ref).max()
