@@ -136,6 +136,26 @@ def marginal_variances(prob, table_fn, theta):
     return latent, obs
 
 
+def hessian_fd(prob, table_fn, theta, eps=5e-3):
+    """Central-difference Hessian of f with the stencil of
+    PyINLA._evaluate_hessian_f (pyinla.py:767-910)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    d = theta.size
+    E = eps * np.eye(d)
+    f0 = evaluate_f(prob, table_fn, theta)
+    H = np.zeros((d, d))
+    for i in range(d):
+        H[i, i] = (evaluate_f(prob, table_fn, theta + E[i]) - 2.0 * f0
+                   + evaluate_f(prob, table_fn, theta - E[i])) / eps ** 2
+        for j in range(i + 1, d):
+            fpp = evaluate_f(prob, table_fn, theta + E[i] + E[j])
+            fpm = evaluate_f(prob, table_fn, theta + E[i] - E[j])
+            fmp = evaluate_f(prob, table_fn, theta - E[i] + E[j])
+            fmm = evaluate_f(prob, table_fn, theta - E[i] - E[j])
+            H[i, j] = H[j, i] = (fpp - fpm - fmp + fmm) / (4.0 * eps * eps)
+    return H
+
+
 def fd_gradient(prob, table_fn, theta, eps=1e-3):
     """Central FD with the reference's evaluation order (pyinla.py:540-580)."""
     theta = np.asarray(theta, dtype=np.float64)

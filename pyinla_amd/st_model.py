@@ -318,14 +318,29 @@ class STGaussianModel:
         """The same FD gradient with the factorizations of up to ``wave``
         stencil points (default all 2d+1) in one batched, graph-replayed launch
         sequence; one extra solver pair per stencil point of a wave."""
-        from .batch import BatchedFactorization
         theta = np.asarray(theta, dtype=np.float64)
         d = theta.size
-        npts = 2 * d + 1
-        wave = npts if wave is None else max(1, min(wave, npts))
         mat = self._stencil(theta, eps)
+        fv = self.f_points(mat, wave=wave, order=list(range(2 * d, -1, -1)))
+        return fv[0], (fv[1:1 + d] - fv[d + 1:]) / (2 * eps)
+
+    def f_points(self, mat, wave: int | None = None, order=None):
+        """f at every column of ``mat`` (dim theta x npts), the factorizations
+        of up to ``wave`` points (default all) in one batched launch sequence;
+        ``order`` is the evaluation order of the columns (default: as given)."""
+        from .batch import BatchedFactorization
+        mat = np.asarray(mat, dtype=np.float64)
+        npts = mat.shape[1]
+        if wave is None:
+            # as many extra solver pairs (factor blocks and block inverses of
+            # both matrices) as fit in 70 % of the free device memory
+            free, _ = torch.cuda.mem_get_info(self.device)
+            per_pair = 8 * 2 * 3 * self.n * self.b * self.b
+            wave = 1 + int(0.7 * free // per_pair)
+        wave = max(1, min(wave, npts))
         pool = getattr(self, "_fd_pool", None)
         if pool is None or len(pool[0]) != wave:
+            self._fd_pool = None
             n, b, a = self.n, self.b, self.a
             conds = [self.cond] + [SerinvSolver(None, b, n, a, device=self.device) for _ in range(wave - 1)]
             priors = [self.prior] + [SerinvSolver(None, b, n, 0, device=self.device) for _ in range(wave - 1)]
@@ -333,7 +348,7 @@ class STGaussianModel:
             self._fd_pool = (conds, priors, batch)
         conds, priors, batch = self._fd_pool
         fv = np.zeros(npts)
-        order = list(range(npts - 1, -1, -1))
+        order = list(range(npts)) if order is None else list(order)
         for w0 in range(0, npts, wave):
             pts = order[w0:w0 + wave]
             for j, k in enumerate(pts):
@@ -351,7 +366,67 @@ class STGaussianModel:
                     raise np.linalg.LinAlgError("not positive definite in the FD batch")
                 quad = float(torch.dot(self.x, self.rhs))
                 fv[k] = self._tail(th, ld_c, ld_p, quad)["f"]
-        return fv[0], (fv[1:1 + d] - fv[d + 1:]) / (2 * eps)
+        return fv
+
+    @staticmethod
+    def _hessian_stencil(theta, eps):
+        """Points of PyINLA._evaluate_hessian_f (pyinla.py:767-910): theta,
+        theta +- eps e_i, and theta +- eps e_i +- eps e_j for i < j."""
+        d = theta.size
+        E = eps * np.eye(d)
+        pts = [theta.copy()]
+        for i in range(d):
+            pts += [theta + E[i], theta - E[i]]
+        for i in range(d):
+            for j in range(i + 1, d):
+                pts += [theta + E[i] + E[j], theta + E[i] - E[j], theta - E[i] + E[j], theta - E[i] - E[j]]
+        return np.stack(pts, axis=1)
+
+    def hessian_fd(self, theta, eps: float = 5e-3, batched: bool = True, wave: int | None = None):
+        """Central-difference Hessian of f as PyINLA._evaluate_hessian_f
+        computes it (pyinla.py:767-910, default eps_hessian_f = 5e-3,
+        pyinla_config.py:52): 1 + 2d + 2d(d-1) evaluations of f, batched (the
+        factorizations of ``wave`` points per launch sequence) or one by one."""
+        theta = np.asarray(theta, dtype=np.float64)
+        d = theta.size
+        mat = self._hessian_stencil(theta, eps)
+        if batched:
+            fv = self.f_points(mat, wave=wave)
+        else:
+            fv = np.array([self.f(mat[:, k]) for k in range(mat.shape[1])])
+        f0 = fv[0]
+        H = np.zeros((d, d))
+        for i in range(d):
+            H[i, i] = (fv[1 + 2 * i] - 2.0 * f0 + fv[2 + 2 * i]) / eps ** 2
+        q = 1 + 2 * d
+        for i in range(d):
+            for j in range(i + 1, d):
+                fpp, fpm, fmp, fmm = fv[q:q + 4]
+                H[i, j] = H[j, i] = (fpp - fpm - fmp + fmm) / (4.0 * eps * eps)
+                q += 4
+        return H
+
+    def hessian_adjoint(self, theta, eps: float = 5e-3):
+        """Hessian of f from central differences of the adjoint gradient:
+        2d evaluations of value_and_grad instead of 1 + 2d + 2d(d-1) of f,
+        symmetrised (no reference counterpart)."""
+        theta = np.asarray(theta, dtype=np.float64)
+        d = theta.size
+        H = np.zeros((d, d))
+        for i in range(d):
+            e = np.zeros(d)
+            e[i] = eps
+            _, gp = self.value_and_grad(theta + e)
+            _, gm = self.value_and_grad(theta - e)
+            H[:, i] = (gp - gm) / (2.0 * eps)
+        return 0.5 * (H + H.T)
+
+    def cov_theta(self, theta, eps: float = 5e-3, method: str = "fd"):
+        """Covariance of theta as the inverse of the Hessian of f
+        (PyINLA.compute_covariance_hp, pyinla.py:747-765); ``method`` "fd"
+        (the reference's stencil) or "adjoint" (differences of gradients)."""
+        H = self.hessian_fd(theta, eps) if method == "fd" else self.hessian_adjoint(theta, eps)
+        return np.linalg.inv(H)
 
     # ---------------------------------------------------------- adjoint
 

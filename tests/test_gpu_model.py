@@ -208,3 +208,22 @@ def test_marginal_variances_match_oracle(cuda, small, kind):
         assert _rel(v.cpu().numpy(), r) < 1e-9
     ref = om.evaluate_f(prob, cm.table, prob.theta, return_parts=True)
     assert abs(model.f(prob.theta) - ref["f"]) <= 1e-10 * max(abs(ref["ld_c"]), abs(ref["f"]))
+
+
+def test_hessian_and_cov_theta(cuda, small):
+    """FD Hessian (reference stencil, batched and one by one) against the
+    oracle's; the Hessian from differences of adjoint gradients against the
+    exact Hessian of the dense torch f; cov_theta = inverse of the Hessian."""
+    cm = CoefficientModel("baseline")
+    model = STGaussianModel(small, device=cuda)
+    th = np.asarray(small.theta)
+    H_ref = om.hessian_fd(small, cm.table, th, eps=5e-3)
+    for batched in (True, False):
+        H = model.hessian_fd(th, eps=5e-3, batched=batched, wave=7)
+        assert _rel(H, H_ref) < 1e-6
+    Hx = torch.autograd.functional.hessian(lambda t: om.dense_f_torch(small, baseline_table, t),
+                                           torch.tensor(th, dtype=torch.float64)).numpy()
+    Ha = model.hessian_adjoint(th, eps=5e-3)
+    assert _rel(Ha, Hx) < 1e-4
+    C = model.cov_theta(th, method="fd")
+    assert np.allclose(C @ H_ref, np.eye(th.size), atol=1e-6)

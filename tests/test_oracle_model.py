@@ -118,3 +118,20 @@ def test_marginal_variances_against_dense_inverse(kind):
     for (A, _, _), v in zip(om._obs(p), obs):
         ref = np.einsum("ij,jk,ik->i", A.toarray(), S, A.toarray())
         assert np.abs(v - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_hessian_fd_against_dense_autograd():
+    """The reference-stencil FD Hessian of the oracle f against the exact
+    Hessian of the dense torch f (second-order autograd); the FD truncation
+    error at eps = 5e-3 bounds the agreement."""
+    p = st_problem(nt=5, nx=4, ny=4, nb=3, seed=6)
+    cm = CoefficientModel("baseline")
+    H = om.hessian_fd(p, cm.table, p.theta, eps=5e-3)
+    th = torch.tensor(p.theta, dtype=torch.float64)
+    Hx = torch.autograd.functional.hessian(lambda t: om.dense_f_torch(p, baseline_table, t), th).numpy()
+    assert np.allclose(H, H.T)
+    # O(eps^2) truncation: 1.4e-6 at eps = 5e-3, a quarter of it at half the step
+    err = np.abs(H - Hx).max() / np.abs(Hx).max()
+    assert err <= 1e-5
+    H2 = om.hessian_fd(p, cm.table, p.theta, eps=2.5e-3)
+    assert np.abs(H2 - Hx).max() / np.abs(Hx).max() <= 0.4 * err
