@@ -145,9 +145,20 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     ls.push_back(std::move(l0));
   }
   // The recurrence, two dependent launches per block (i = n-1 .. 0):
-  //   Sigma_{i+1,i} = -Sigma_{i+1,i+1} G_i - Sigma_{n,i+1}^T Ga_i   (into L_i)
-  //   Sigma_{n,i}   = -Sigma_{n,i+1} G_i - Sigma_T Ga_i            (into B_i)
-  //   Sigma_ii     +=  - G_i^T Sigma_{i+1,i} - Ga_i^T Sigma_{n,i}  (into D_i)
+  //   l2(i): Sigma_{i+1,i} = -Sigma_{i+1,i+1} G_i - Sigma_{n,i+1}^T Ga_i   (into L_i)
+  //   l3(i): Sigma_ii     +=  - G_i^T Sigma_{i+1,i} - Ga_i^T Sigma_{n,i}  (into D_i)
+  //          Sigma_{n,i-1} = -Sigma_{n,i} G_{i-1} - Sigma_T Ga_{i-1}     (into B_{i-1})
+  // Sigma_{n,i-1} needs only Sigma_{n,i}, so it rides in l3(i), whose
+  // lower-triangle tiles fill half of the resident slots, and l2 keeps its
+  // b x b tiles (one full wave at b = 2000).  Sigma_{n,n-1} = -Sigma_T Ga_{n-1}
+  // is l2(n-1).  Ga_i = B_i W_i was formed in l0, so B_{i-1} is free.
+  auto arrow_job = [&](int i) {  // Sigma_{n,i}
+    double* Bi = B + (int64_t)i * a * b;
+    GemmJob s2 = job(Bi, a, b, b, 0.0);
+    if (i < n - 1) add_seg(s2, seg(B + (int64_t)(i + 1) * a * b, b, 0, Gall + (int64_t)i * b * b, b, 0, b, -1.0));
+    add_seg(s2, seg(T, a, 0, Gaall + (int64_t)i * a * b, b, 0, a, -1.0));
+    return s2;
+  };
   for (int i = n - 1; i >= 0; --i) {
     const bool hasL = i < n - 1;
     double* Di = D + (int64_t)i * b * b;
@@ -158,28 +169,24 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     double* G = hasL ? Gall + (int64_t)i * b * b : nullptr;
     double* Ga = a > 0 ? Gaall + (int64_t)i * a * b : nullptr;
     GemmLaunch l2(ts), l3(ts);
-    // the few Sigma_{n,i} tiles go first: dispatched ahead of the b x b
-    // tiles they start at once instead of trailing the launch as a second wave
-    if (a > 0) {
-      GemmJob s2 = job(Bi, a, b, b, 0.0);  // Sigma_{n,i}
-      if (hasL) add_seg(s2, seg(Bn, b, 0, G, b, 0, b, -1.0));
-      add_seg(s2, seg(T, a, 0, Ga, b, 0, a, -1.0));
-      l2.add(s2);
-    }
+    if (a > 0 && !hasL) l2.add(arrow_job(i));
     if (hasL) {
       GemmJob s2 = job(Li, b, b, b, 0.0);  // Sigma_{i+1,i}
       add_seg(s2, seg(Dn, b, 0, G, b, 0, b, -1.0));
       if (a > 0) add_seg(s2, seg(Bn, b, 1, Ga, b, 0, a, -1.0));
       l2.add(s2);
     }
+    // the few Sigma_{n,i-1} tiles go first: dispatched ahead of the b x b
+    // tiles they start at once instead of trailing the launch
+    if (a > 0 && i > 0) l3.add(arrow_job(i - 1));
     if (hasL || a > 0) {
       GemmJob sd = job(Di, b, b, b, 1.0, /*lower=*/1, /*mirror=*/1);  // Sigma_ii
       if (hasL) add_seg(sd, seg(G, b, 1, Li, b, 0, b, -1.0));
       if (a > 0) add_seg(sd, seg(Ga, b, 1, Bi, b, 0, a, -1.0));
       l3.add(sd);
     }
-    ls.push_back(std::move(l2));
-    ls.push_back(std::move(l3));
+    if (l2.tiles() > 0) ls.push_back(std::move(l2));
+    if (l3.tiles() > 0) ls.push_back(std::move(l3));
   }
   ws.potri_plan.reset(new DevPlan());
   rc = ws.potri_plan->build(ls);
