@@ -22,8 +22,8 @@ def _rel(x, y):
     return float(np.abs(np.asarray(x) - np.asarray(y)).max() / max(np.abs(y).max(), 1e-300))
 
 
-CASES = [(1, 1, 0), (3, 17, 3), (3, 64, 4), (4, 92, 4), (2, 130, 0), (3, 200, 9), (50, 92, 4),
-         (3, 600, 5), (2, 640, 0)]  # b >= 512 exercises the 128x128-tile GEMM
+CASES = [(1, 1, 0), (1, 40, 3), (2, 70, 2), (3, 17, 3), (3, 64, 4), (4, 92, 4), (2, 130, 0), (3, 200, 9),
+         (50, 92, 4), (3, 100, 64), (3, 600, 5), (2, 640, 0)]  # b >= 512 exercises the 128x128-tile GEMM
 
 
 @pytest.mark.parametrize("n,b,a", CASES)
