@@ -462,21 +462,28 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemv_n(GemvJob g) {
       const bool vec = sg.ldx == 1 && g.nrhs == 1 && (sg.lda & 1) == 0 &&
                        ((reinterpret_cast<uintptr_t>(sg.A) | reinterpret_cast<uintptr_t>(sg.x)) & 15) == 0;
       if (vec) {
-        const int k0 = kmin & ~1;
-        for (int base = k0; base < kmax; base += 512) {
+        // pairs [k0, ke2) as double2; all 16 loads of a 2048-wide stretch are
+        // issued before the first FMA (addresses clamped into the row, the
+        // FMAs masked), so a row of up to 2048 costs one memory round trip
+        const int k0 = kmin & ~1, ke2 = kmax & ~1;
+        for (int base = k0; base < ke2; base += 2048) {
+          double2 av[16], xv[16];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < 16; ++u) {
+            const int kk = min(base + 128 * u + 2 * lane, ke2 - 2);
+            av[u] = *reinterpret_cast<const double2*>(a + kk);
+            xv[u] = *reinterpret_cast<const double2*>(sg.x + kk);
+          }
+#pragma unroll
+          for (int u = 0; u < 16; ++u) {
             const int kk = base + 128 * u + 2 * lane;
-            if (kk + 1 < kmax) {
-              const double2 av = *reinterpret_cast<const double2*>(a + kk);
-              const double2 xv = *reinterpret_cast<const double2*>(sg.x + kk);
-              p[u] = fma(av.y, xv.y, p[u]);
-              if (kk >= kmin) p[u] = fma(av.x, xv.x, p[u]);
-            } else if (kk < kmax && kk >= kmin) {
-              p[u] = fma(a[kk], sg.x[kk], p[u]);
+            if (kk < ke2) {
+              const double t = fma(av[u].y, xv[u].y, p[u & 3]);
+              p[u & 3] = (kk >= kmin) ? fma(av[u].x, xv[u].x, t) : t;
             }
           }
         }
+        if ((kmax & 1) && kmax - 1 >= kmin && lane == 0) p[0] = fma(a[kmax - 1], sg.x[kmax - 1], p[0]);
       } else {
         for (int base = kmin; base < kmax; base += 256) {
 #pragma unroll
@@ -519,14 +526,26 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemv_t_part(GemvJob g, double* 
   if (sg.tri == 1) khi = min(khi, j + 1);
   if (sg.tri == 2) klo = max(klo, j);
   const bool col = j < g.m;
+  const int jc = min(j, g.m - 1);
+  const int wu = __builtin_amdgcn_readfirstlane(w);
   for (int rc = 0; rc < g.nrhs; ++rc) {
     double p[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int base = kb + w; base < ke; base += 16) {
+    // wave w takes rows kb + w + 4r (r < 32) in two batches of 16 whose loads
+    // are all issued before their FMAs (addresses clamped into the chunk,
+    // the FMAs masked)
+#pragma unroll 1
+    for (int h = 0; h < GEMV_TK / 64; ++h) {
+      double av[16], xv[16];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = base + 4 * u;
-        if (col && k < ke && k >= klo && k < khi)
-          p[u] = fma(sg.A[(int64_t)k * sg.lda + j], sg.x[(int64_t)k * sg.ldx + rc], p[u]);
+      for (int r = 0; r < 16; ++r) {
+        const int kc = min(kb + wu + 4 * (16 * h + r), ke - 1);  // uniform: x from scalar loads
+        av[r] = sg.A[(int64_t)kc * sg.lda + jc];
+        xv[r] = sg.x[(int64_t)kc * sg.ldx + rc];
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int k = kb + w + 4 * (16 * h + r);
+        if (col && k < ke && k >= klo && k < khi) p[r & 3] = fma(av[r], xv[r], p[r & 3]);
       }
     }
     red[w][lane] = (p[0] + p[1]) + (p[2] + p[3]);
