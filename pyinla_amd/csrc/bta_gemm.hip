@@ -528,13 +528,16 @@ __global__ void __launch_bounds__(BTA_THREADS) k_gemv_t_part(GemvJob g, double* 
   const bool col = j < g.m;
   const int jc = min(j, g.m - 1);
   const int wu = __builtin_amdgcn_readfirstlane(w);
+  // a chunk outside the triangle for every column of this workgroup loads nothing
+  const int j0 = blockIdx.x * 64, j1 = min(j0 + 64, g.m) - 1;
+  const int nh = ((sg.tri == 1 && kb > j1) || (sg.tri == 2 && ke - 1 < j0)) ? 0 : GEMV_TK / 64;
   for (int rc = 0; rc < g.nrhs; ++rc) {
     double p[4] = {0.0, 0.0, 0.0, 0.0};
     // wave w takes rows kb + w + 4r (r < 32) in two batches of 16 whose loads
     // are all issued before their FMAs (addresses clamped into the chunk,
     // the FMAs masked)
 #pragma unroll 1
-    for (int h = 0; h < GEMV_TK / 64; ++h) {
+    for (int h = 0; h < nh; ++h) {
       double av[16], xv[16];
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
