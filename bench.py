@@ -320,7 +320,7 @@ def main():
 
     # live per-kernel-class timing (HIP events on the launch stream) of one
     # adjoint evaluation
-    classes = {0: "panel", 1: "update", 2: "grouped", 3: "gemv"}
+    classes = {0: "panel", 1: "update", 2: "grouped", 3: "gemv", 4: "blas"}
     tot = {c: 0.0 for c in classes}
     cnt = {c: 0 for c in classes}
     import ctypes as C
@@ -343,10 +343,13 @@ def main():
         flops_cls = {0: flops_potrf(n, b, a) + flops_potrf(n, b, 0), 1: 0.0}
     else:
         flops_cls = {0: fp_c + fp_p, 1: fu_c + fu_p}
-    flops_cls[2] = flops_potri(n, b, a) + flops_potri(n, b, 0)
+    # the rocBLAS dgemm calls (class 4) are the Sigma_{i+1,i} products, 2 b^3 each
+    flops_cls[4] = cnt[4] * 2.0 * b**3
+    flops_cls[2] = flops_potri(n, b, a) + flops_potri(n, b, 0) - flops_cls[4]
     potrf_ms = tot[0] + tot[1]
     potrf_tflops = (flops_potrf(n, b, a) + flops_potrf(n, b, 0)) / (potrf_ms * 1e-3) / 1e12 if potrf_ms else None
-    potri_tflops = flops_cls[2] / (tot[2] * 1e-3) / 1e12 if tot[2] else None
+    potri_ms = tot[2] + tot[4]
+    potri_tflops = (flops_cls[2] + flops_cls[4]) / (potri_ms * 1e-3) / 1e12 if potri_ms else None
     dom = max(classes, key=lambda c: tot[c])
     if dom == 3:
         ach = bytes_potrs(n, b, a) / (tot[3] * 1e-3 / max(cnt[3], 1) * cnt[3]) / 1e9
@@ -359,7 +362,7 @@ def main():
         roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_FP64_TFLOPS, "traffic": None}
     roof["kernel"] = {0: "k_panel_flow" if flow else "k_panel_step", 1: "k_gemm_nt_sub", 2: "k_gemm_grouped",
-                      3: "k_gemv"}[dom]
+                      3: "k_gemv", 4: "rocblas_dgemm"}[dom]
     # HBM bytes per launch of that kernel from the committed rocprofv3 PMC
     # summary (FETCH_SIZE / WRITE_SIZE, scaled by a 1 GiB copy), if present
     tr_path = ROOT / "profiles" / "pmc_traffic.json"

@@ -37,6 +37,7 @@ extern "C" {
 #define BTA_E_ARG (-1)      /* invalid shape / pointer / option */
 #define BTA_E_SHAPE (-2)    /* shape does not match the workspace */
 #define BTA_E_TIMEOUT (-3)  /* a bounded in-kernel wait timed out */
+#define BTA_E_LIB (-4)      /* a rocBLAS call failed */
 
 typedef struct bta_ws* bta_ws_t;
 
@@ -117,6 +118,11 @@ double* bta_ws_logdet_ptr(bta_ws_t ws);
  * concurrently per wave (taken from the first problem's workspace; 0 = 6,
  * at most 8). */
 #define BTA_OPT_BATCH_LANES 6
+/* Selected inversion: 1 (default) = the b x b x b products
+ * Sigma_{i+1,i} = -Sigma_{i+1,i+1} G_i as rocBLAS dgemm calls on the caller's
+ * stream (not while that stream is under graph capture), 0 = the grouped
+ * FP64-MFMA kernel for every product. */
+#define BTA_OPT_BLAS 7
 int bta_ws_set_option(bta_ws_t ws, int option, int value);
 
 /* Host-only diagnostic (no reference counterpart): the helper task lists of
@@ -132,8 +138,8 @@ int bta_flow_schedule(int nT, int nTL, int nTa, int helpers, int delta, uint32_t
  * calls with perf_counter, structured_solver.py:80-100).  While on, graphs are
  * bypassed and every launch of the classes 0 = panel factorization step,
  * 1 = block update GEMM, 2 = grouped GEMM (inverses / selected inversion),
- * 3 = GEMV sweep, 4 = other is bracketed by a pair of hipEvents on its stream.
- * bta_ws_get_timing synchronises and returns the summed milliseconds and the
+ * 3 = GEMV sweep, 4 = rocBLAS dgemm of the selected inversion is bracketed
+ * by a pair of hipEvents on its stream.  bta_ws_get_timing synchronises and returns the summed milliseconds and the
  * number of launches of a class since timing was switched on or reset. */
 int bta_ws_set_timing(bta_ws_t ws, int on);
 int bta_ws_reset_timing(bta_ws_t ws);

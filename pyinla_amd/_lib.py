@@ -43,7 +43,8 @@ SIGNATURES = {
     "bta_flow_schedule": (_I, [_I, _I, _I, _I, _I, _P, C.c_int64, _P, C.POINTER(C.c_int64)]),
 }
 
-BTA_E = {-1: "invalid argument", -2: "shape does not match the workspace", -3: "in-kernel wait timed out"}
+BTA_E = {-1: "invalid argument", -2: "shape does not match the workspace", -3: "in-kernel wait timed out",
+         -4: "rocBLAS call failed"}
 
 _lib = None
 

@@ -1,4 +1,6 @@
 // extern "C" entry points of libpyinla_bta.so (declared in include/pyinla_bta.h).
+#include <rocblas/rocblas.h>
+
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -157,6 +159,8 @@ int bta_ws_destroy(bta_ws_t ws) {
   if (ws->gstream) hipStreamDestroy(ws->gstream);
   ws->w.trtri_plan.reset();
   ws->w.potri_plan.reset();
+  ws->w.potri_plan_blas.reset();
+  if (ws->w.blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ws->w.blas));
   ws->w.update_plan.reset();
   hipFree(ws->w.winv);
   if (ws->w.gfull) hipFree(ws->w.gfull);
@@ -369,6 +373,7 @@ int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   if (option == BTA_OPT_FLOW_HELPERS && (value < 0 || value > 1023)) return BTA_E_ARG;
   if (option == BTA_OPT_FLOW_DELTA && (value < 0 || value > 64)) return BTA_E_ARG;
   if (option == BTA_OPT_BATCH_LANES && (value < 0 || value > 512)) return BTA_E_ARG;
+  if (option == BTA_OPT_BLAS && value != 0 && value != 1) return BTA_E_ARG;
   std::lock_guard<std::mutex> g(ws->mu);
   if (option == BTA_OPT_TILE_UPDATE) ws->w.tile_update = value;
   else if (option == BTA_OPT_TILE_GROUPED) ws->w.tile_grouped = value;
@@ -376,6 +381,10 @@ int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   else if (option == BTA_OPT_FLOW_HELPERS) ws->w.flow_helpers = value;
   else if (option == BTA_OPT_FLOW_DELTA) ws->w.flow_delta = value;
   else if (option == BTA_OPT_BATCH_LANES) ws->w.batch_lanes = value;
+  else if (option == BTA_OPT_BLAS) {
+    ws->w.use_blas = value;
+    ws->w.potri_plan.reset();  // rebuilt with or without the dgemm calls
+  }
   else return BTA_E_ARG;
   // recorded graphs embed the old launch shapes
   for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);

@@ -21,7 +21,7 @@ enum KernelClass : int {
   KC_UPDATE = 1,    // k_gemm_nt_sub (block updates D_{i+1}, B_{i+1}, T)
   KC_GROUPED = 2,   // k_gemm_grouped (block inverses, selected inversion)
   KC_GEMV = 3,      // GEMV kernels (triangular sweeps)
-  KC_OTHER = 4,
+  KC_BLAS = 4,      // rocBLAS dgemm of the selected inversion (Sigma_{i+1,i} products)
   KC_COUNT = 5
 };
 
@@ -96,6 +96,22 @@ struct Workspace {
   const double* w_of = nullptr;    // D pointer W was computed from
   // cached launch plans, keyed by the block pointers they were built for
   std::unique_ptr<DevPlan> trtri_plan, potri_plan, update_plan;
+  // Selected inversion with the b x b x b products Sigma_{i+1,i} = -Sigma_{i+1,i+1} G_i
+  // left to rocBLAS dgemm (faster than the grouped kernel's NN form on MI355X):
+  // its grouped launches, and the order of grouped launches (>= 0) and
+  // dgemm calls (-1 - index) in the recurrence.  potri_plan is kept for streams
+  // under graph capture.
+  struct BlasGemm {
+    const double* A;  // Sigma_{i+1,i+1}
+    const double* B;  // G_i
+    double* C;        // Sigma_{i+1,i}
+    double beta;      // 1 when the arrow term is already in C
+  };
+  std::unique_ptr<DevPlan> potri_plan_blas;
+  std::vector<BlasGemm> potri_blas;
+  std::vector<int> potri_blas_order;
+  void* blas = nullptr;  // rocblas_handle, created with the first such plan
+  int use_blas = 1;      // 0: grouped kernel for every product
   const void* update_key[4] = {nullptr, nullptr, nullptr, nullptr};
   int update_a = -1;
   const void* trtri_key = nullptr;

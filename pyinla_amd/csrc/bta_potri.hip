@@ -18,7 +18,10 @@
 // recurrence: one launch computes them for every block at once (full
 // occupancy), leaving two dependent launches per block.  Sigma_ii is written
 // full symmetric.
+#include <rocblas/rocblas.h>
+
 #include "bta_internal.h"
+#include "../../include/pyinla_bta.h"
 
 namespace bta {
 
@@ -194,6 +197,65 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     ws.potri_plan.reset();
     return rc;
   }
+  ws.potri_plan_blas.reset();
+  ws.potri_blas.clear();
+  ws.potri_blas_order.clear();
+  if (ws.use_blas && n > 1) {
+    // The same recurrence with Sigma_{i+1,i} = -Sigma_{i+1,i+1} G_i as one
+    // rocBLAS dgemm per block.  Its arrow term -Sigma_{n,i+1}^T Ga_i is written
+    // into L_i one launch earlier, by l3(i+1) (Sigma_{n,i+1} is final there, L_i
+    // is free once G_i exists), and the dgemm accumulates onto it.
+    if (!ws.blas) {
+      rocblas_handle h;
+      if (rocblas_create_handle(&h) != rocblas_status_success) return BTA_E_LIB;
+      ws.blas = h;
+    }
+    std::vector<GemmLaunch> lb;
+    lb.push_back(ls.front());  // l0
+    ws.potri_blas_order.push_back(0);
+    for (int i = n - 1; i >= 0; --i) {
+      const bool hasL = i < n - 1;
+      double* Di = D + (int64_t)i * b * b;
+      double* Li = hasL ? L + (int64_t)i * b * b : nullptr;
+      double* Bi = a > 0 ? B + (int64_t)i * a * b : nullptr;
+      double* Ga = a > 0 ? Gaall + (int64_t)i * a * b : nullptr;
+      if (a > 0 && !hasL) {
+        GemmLaunch l2(ts);
+        l2.add(arrow_job(i));
+        ws.potri_blas_order.push_back((int)lb.size());
+        lb.push_back(std::move(l2));
+      }
+      if (hasL) {
+        ws.potri_blas_order.push_back(-1 - (int)ws.potri_blas.size());
+        ws.potri_blas.push_back({D + (int64_t)(i + 1) * b * b, Gall + (int64_t)i * b * b, Li, a > 0 ? 1.0 : 0.0});
+      }
+      GemmLaunch l3(ts);
+      if (a > 0 && i > 0) {
+        l3.add(arrow_job(i - 1));
+        GemmJob s2 = job(L + (int64_t)(i - 1) * b * b, b, b, b, 0.0);  // arrow term of Sigma_{i,i-1}
+        add_seg(s2, seg(Bi, b, 1, Gaall + (int64_t)(i - 1) * a * b, b, 0, a, -1.0));
+        l3.add(s2);
+      }
+      if (hasL || a > 0) {
+        GemmJob sd = job(Di, b, b, b, 1.0, /*lower=*/1, /*mirror=*/1);  // Sigma_ii
+        if (hasL) add_seg(sd, seg(Gall + (int64_t)i * b * b, b, 1, Li, b, 0, b, -1.0));
+        if (a > 0) add_seg(sd, seg(Ga, b, 1, Bi, b, 0, a, -1.0));
+        l3.add(sd);
+      }
+      if (l3.tiles() > 0) {
+        ws.potri_blas_order.push_back((int)lb.size());
+        lb.push_back(std::move(l3));
+      }
+    }
+    ws.potri_plan_blas.reset(new DevPlan());
+    rc = ws.potri_plan_blas->build(lb);
+    if (rc) {
+      ws.potri_plan_blas.reset();
+      ws.potri_blas.clear();
+      ws.potri_blas_order.clear();
+      return rc;
+    }
+  }
   for (int q = 0; q < 4; ++q) ws.potri_key[q] = key[q];
   ws.potri_a = a;
   ws.potri_tile = ws.tile_grouped;
@@ -209,9 +271,30 @@ int potri_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     const int rc = trtri_enqueue(ws, D, n, b, st);
     if (rc) return rc;
   }
-  for (size_t q = 0; q < ws.potri_plan->launches.size(); ++q) {
-    TimedScope ts(ws, KC_GROUPED, st);
-    ws.potri_plan->launch((int)q, st);
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
+  if (!ws.potri_plan_blas || cap != hipStreamCaptureStatusNone) {
+    for (size_t q = 0; q < ws.potri_plan->launches.size(); ++q) {
+      TimedScope ts(ws, KC_GROUPED, st);
+      ws.potri_plan->launch((int)q, st);
+    }
+    return (int)hipGetLastError();
+  }
+  rocblas_handle h = static_cast<rocblas_handle>(ws.blas);
+  if (rocblas_set_stream(h, st) != rocblas_status_success) return BTA_E_LIB;
+  const double minus_one = -1.0;
+  for (const int q : ws.potri_blas_order) {
+    if (q >= 0) {
+      TimedScope ts(ws, KC_GROUPED, st);
+      ws.potri_plan_blas->launch(q, st);
+      continue;
+    }
+    const Workspace::BlasGemm& g = ws.potri_blas[-1 - q];
+    TimedScope ts(ws, KC_BLAS, st);
+    // row-major C = A B is column-major C^T = B^T A^T
+    if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, b, b, b, &minus_one, g.B, b, g.A, b,
+                      &g.beta, g.C, b) != rocblas_status_success)
+      return BTA_E_LIB;
   }
   return (int)hipGetLastError();
 }

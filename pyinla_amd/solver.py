@@ -132,6 +132,12 @@ class SerinvSolver:
         counterpart; 0 = automatic)."""
         _lib.check(self.lib.bta_ws_set_option(self._ws, 6, int(lanes)), "set_option")
 
+    def set_selinv_blas(self, on: bool = True) -> None:
+        """Selected inversion: the b x b x b products Sigma_{i+1,i} =
+        -Sigma_{i+1,i+1} G_i as rocBLAS dgemm calls (default) or in the grouped
+        FP64-MFMA kernel with everything else (no reference counterpart)."""
+        _lib.check(self.lib.bta_ws_set_option(self._ws, 7, 1 if on else 0), "set_option")
+
     def status(self):
         """(info, logdet, err) of the last factorization (synchronises)."""
         info = C.c_int64()

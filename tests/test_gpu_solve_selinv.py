@@ -186,6 +186,29 @@ def test_batched_flow_factorization(cuda, force_flow):
                 assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
 
 
+@pytest.mark.parametrize("blas", [False, True])
+@pytest.mark.parametrize("n,b,a", [(2, 70, 2), (4, 92, 4), (3, 130, 0), (3, 300, 6)])
+def test_selected_inversion_blas_and_grouped(cuda, n, b, a, blas):
+    """Both forms of the Sigma_{i+1,i} products (rocBLAS dgemm / grouped
+    kernel) against the oracle, twice in a row on the same solver."""
+    kf = known_factor_bta(n, b, a, seed=300 + n * b + a)
+    s = SerinvSolver(None, b, n, a, use_graph=False)
+    s.set_selinv_blas(blas)
+    D, L = kf.Lf_D.copy(), kf.Lf_L.copy()
+    B = kf.Lf_B.copy() if a else None
+    T = kf.Lf_T.copy() if a else None
+    ob.pobtasi(D, L, B, T)
+    for _ in range(2):
+        s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if a else None, kf.Q_T if a else None)
+        s.factorize("bta")
+        s.selected_inversion("bta")
+        assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
+        assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < RTOL
+        if a:
+            assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
+            assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
+
+
 @pytest.mark.parametrize("n,b,a", [(3, 600, 5), (2, 130, 0), (4, 92, 4), (2, 300, 6)])
 def test_selected_inversion_with_128_tiles(cuda, n, b, a):
     """The 128x128-tile grouped GEMM (block inverses + selected inversion,
