@@ -7,7 +7,10 @@
 //     [[W11, 0], [W21, W22]],  W21 = -W22 (L21 W11),
 // where T1^T = W11^T L21^T is parked in the (still unused) upper triangle of
 // W_i.  Every level is two grouped-GEMM launches over all pairs of all n
-// blocks, so the whole inverse costs ~2 log2(b/64) launches.
+// blocks, so the whole inverse costs ~2 log2(b/64) launches.  The parked
+// blocks stay: every reader of W (the GEMV sweeps, G_i, Ga_i, W_i^T W_i and
+// the merge levels themselves) masks the strict upper triangle element by
+// element, so it is never cleared.
 //
 // Selected inversion then runs the Takahashi recurrence backwards in i with
 // G_i = L_i W_i and Ga_i = B_i W_i:
@@ -33,18 +36,6 @@ __global__ void __launch_bounds__(BTA_THREADS) k_leaf_copy(double* W, const doub
   const int c = threadIdx.x & 63;
   for (int r = threadIdx.x >> 6; r < mv; r += 4)
     if (c < mv) dst[(int64_t)r * b + c] = src[r * 64 + c];
-}
-
-// Zero the strict upper triangle of n b x b blocks (one workgroup per 64-row strip).
-__global__ void __launch_bounds__(BTA_THREADS) k_zero_upper(double* W, int b, int nT) {
-  const int blk = blockIdx.x / nT, rt = blockIdx.x % nT;
-  double* base = W + (int64_t)blk * b * b;
-  const int rows = min(64, b - rt * 64);
-  for (int r = 0; r < rows; ++r) {
-    const int i = rt * 64 + r;
-    double* row = base + (int64_t)i * b;
-    for (int j = i + 1 + threadIdx.x; j < b; j += BTA_THREADS) row[j] = 0.0;
-  }
 }
 
 int trtri_prepare(Workspace& ws, const double* D, int n, int b) {
@@ -96,7 +87,6 @@ int trtri_enqueue(Workspace& ws, const double* D, int n, int b, hipStream_t st) 
     TimedScope ts(ws, KC_GROUPED, st);
     ws.trtri_plan->launch((int)q, st);
   }
-  hipLaunchKernelGGL(k_zero_upper, dim3(n * nT), dim3(BTA_THREADS), 0, st, ws.wfull, b, nT);
   return (int)hipGetLastError();
 }
 
