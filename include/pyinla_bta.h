@@ -159,7 +159,10 @@ int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
  * blocks of Sigma = Q^-1 on the BTA pattern -- D[i] <- Sigma_ii (full
  * symmetric), L[i] <- Sigma_{i+1,i}, B[i] <- Sigma_{n,i}, T <- Sigma_T.
  * Replaces serinv.pobtasi(D, L, B, T) / pobtsi(D, L) as called by
- * SerinvSolver.selected_inversion (structured_solver.py:168-190). */
+ * SerinvSolver.selected_inversion (structured_solver.py:168-190).  With
+ * BTA_OPT_BLAS on (the default) the Sigma_{i+1,i} products are rocBLAS dgemm
+ * calls on `stream`; everything else runs in this library's kernels.  Returns
+ * BTA_E_LIB if a rocBLAS call fails. */
 int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
               void* stream);
 
