@@ -186,13 +186,16 @@ def test_batched_flow_factorization(cuda, force_flow):
                 assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("blas", [False, True])
 @pytest.mark.parametrize("n,b,a", [(2, 70, 2), (4, 92, 4), (3, 130, 0), (3, 300, 6)])
-def test_selected_inversion_blas_and_grouped(cuda, n, b, a, blas):
+def test_selected_inversion_blas_and_grouped(cuda, n, b, a, blas, use_graph):
     """Both forms of the Sigma_{i+1,i} products (rocBLAS dgemm / grouped
-    kernel) against the oracle, twice in a row on the same solver."""
+    kernel) against the oracle, twice in a row on the same solver; with
+    use_graph the second call replays the recorded graph, which holds the
+    all-grouped plan (rocBLAS is not called under capture)."""
     kf = known_factor_bta(n, b, a, seed=300 + n * b + a)
-    s = SerinvSolver(None, b, n, a, use_graph=False)
+    s = SerinvSolver(None, b, n, a, use_graph=use_graph)
     s.set_selinv_blas(blas)
     D, L = kf.Lf_D.copy(), kf.Lf_L.copy()
     B = kf.Lf_B.copy() if a else None
