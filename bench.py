@@ -174,7 +174,7 @@ def bench_mixed_batch(args, dev):
         ld_exact.append(2.0 * (torch.log(torch.diagonal(Lf[0], dim1=1, dim2=2)).sum()
                                + torch.log(torch.diag(Lf[3])).sum()).item())
         del Lf
-        s = SerinvSolver(None, b, n, a, device=dev)
+        s = SerinvSolver(None, b, n, a, device=dev, reserve="factor")
         Qs.append(Q)
         solvers.append(s)
         total += flops_potrf(n, b, a)

@@ -19,7 +19,21 @@
  *   - stream is a hipStream_t passed as void* (NULL = default stream).
  *   - Every function returns 0 on success, a negative BTA_E_* code on invalid
  *     arguments, or a positive hipError_t code from the runtime.
- *   - No function allocates device memory except bta_ws_create.
+ *   - Device memory.  Every buffer the arithmetic touches is allocated by
+ *     bta_ws_create or bta_ws_reserve and counted by bta_ws_bytes /
+ *     bta_ws_allocated: the per-tile inverses, flags and scratch, the
+ *     persistent panel's exchange buffers, and (reserved on request) the block
+ *     inverses W_i (n b^2 doubles) and the selected inversion's G_i / Ga_i
+ *     (n b^2 + n max(a,1) b doubles).  The only other device allocations are
+ *     launch plans -- job tables and task lists of a few hundred KB -- built
+ *     the first time a call sees a new set of block pointers (or ahead of time
+ *     by bta_ws_prepare) and kept with the workspace (at most 16 per kind; a
+ *     full cache is emptied, which synchronises the device and drops the
+ *     recorded graphs).  bta_potri additionally creates a rocBLAS handle on
+ *     its first call with BTA_OPT_BLAS on.  Repeated calls with the same
+ *     pointers allocate nothing.
+ *   - A solve or selected inversion on a workspace without the reservation it
+ *     needs returns BTA_E_NORES (no hidden allocation).
  *   - Calls are asynchronous on the stream; results that a host needs
  *     (logdet, info) are read with bta_ws_status, which synchronises.
  */
@@ -38,23 +52,41 @@ extern "C" {
 #define BTA_E_SHAPE (-2)    /* shape does not match the workspace */
 #define BTA_E_TIMEOUT (-3)  /* a bounded in-kernel wait timed out */
 #define BTA_E_LIB (-4)      /* a rocBLAS call failed */
+#define BTA_E_NORES (-5)    /* the workspace lacks the reservation the call needs */
 
 typedef struct bta_ws* bta_ws_t;
 
 /* Library version (major*10000 + minor*100 + patch). */
 int bta_version(void);
 
-/* Device workspace for one solver instance of shape (n, b, a).  Holds the
- * inverse of every 64x64 diagonal tile of the factor, readiness flags, the
- * log-determinant partials and the scratch used by solve / selected
- * inversion.  Replaces the buffer ownership of SerinvSolver.__init__
- * (structured_solver.py:22-70); the block buffers themselves are owned by the
- * caller.  use_graph != 0 captures each (operation, pointers) sequence once in
- * a hipGraph and replays it. */
-int bta_ws_create(int device, int n, int b, int a, int use_graph, bta_ws_t* out);
+/* Workspace flags (bta_ws_create, bta_ws_reserve, bta_ws_bytes). */
+#define BTA_WS_GRAPH 1   /* capture each (operation, pointers) sequence once in a hipGraph and replay it */
+#define BTA_WS_SOLVE 2   /* reserve the block inverses W_i used by bta_potrs (n b^2 doubles) */
+#define BTA_WS_SELINV 4  /* reserve W_i and the G_i / Ga_i of bta_potri (2 n b^2 + n max(a,1) b doubles) */
+
+/* Device workspace for one solver instance of shape (n, b, a), a <= 64.  Holds
+ * the inverse of every 64x64 diagonal tile of the factor, readiness flags, the
+ * log-determinant partials, the scratch of the solves, the persistent panel's
+ * exchange buffers (b > 64) and the buffers `flags` reserves.  Replaces the
+ * buffer ownership of SerinvSolver.__init__ (structured_solver.py:22-70),
+ * which allocates everything once; the block buffers themselves are owned by
+ * the caller. */
+int bta_ws_create(int device, int n, int b, int a, int flags, bta_ws_t* out);
 int bta_ws_destroy(bta_ws_t ws);
-/* Bytes bta_ws_create allocates for (n, b, a). */
-size_t bta_ws_bytes(int n, int b, int a);
+/* Reserve more buffers (BTA_WS_SOLVE / BTA_WS_SELINV) on an existing
+ * workspace; a no-op for what it already holds.  Not a hot call. */
+int bta_ws_reserve(bta_ws_t ws, int flags);
+/* Bytes bta_ws_create allocates for (n, b, a, flags), plans excluded. */
+size_t bta_ws_bytes(int n, int b, int a, int flags);
+/* Bytes the workspace holds now: buffers, reservations and cached plans.
+ * Replaces SerinvSolver.get_solver_memory's count of solver-owned memory
+ * (structured_solver.py:540-550). */
+size_t bta_ws_allocated(bta_ws_t ws);
+/* Build ahead of time the launch plans of the operations in `ops` (1 =
+ * bta_potrf, 2 = bta_potrs, 4 = bta_potri; bits may be combined) for these
+ * block pointers, so that the calls that follow allocate nothing (e.g. before
+ * graph capture).  No reference counterpart. */
+int bta_ws_prepare(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a, int ops);
 
 /* In-place BTA (a > 0) or BT (a == 0) Cholesky, Q = LL^T, with the
  * log-determinant accumulated on the fly.  Replaces serinv.pobtaf(D, L, B, T)
@@ -94,6 +126,13 @@ int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* st
 /* Device address of the FP64 log-determinant written by the last bta_potrf
  * (valid after the stream reaches that point; no host synchronisation). */
 double* bta_ws_logdet_ptr(bta_ws_t ws);
+
+/* out[0] (device) = 2 sum_i sum_j log D_i[j][j] (+ 2 sum_j log T[j][j] if
+ * a > 0) of the blocks as they are held now, summed in a fixed order.  This is
+ * SerinvSolver.logdet(sparsity) (structured_solver.py:151-166): it reads the
+ * current blocks, so after a BTA factorization a == 0 excludes the tip, as
+ * logdet("bt") does, and it may follow any factorization (a == 0 or not). */
+int bta_logdet(bta_ws_t ws, const double* D, const double* T, int n, int b, int a, double* out, void* stream);
 
 /* Tuning options (no reference counterpart): the output-tile edge (64 or
  * 128; 0 = built-in heuristic) of the FP64-MFMA GEMMs used for the block
@@ -151,7 +190,9 @@ int bta_ws_get_timing(bta_ws_t ws, int kclass, double* total_ms, int64_t* count)
  * replaces SerinvSolver.solve (structured_solver.py:102-149).  a == 0 gives
  * pobts.  rhs is (n*b + a) x nrhs, row-major, leading dimension ldrhs
  * (ldrhs >= nrhs, any nrhs).  The first call after a factorization
- * also forms W_i = D_i^-1 for every block (kept in the workspace). */
+ * also forms W_i = D_i^-1 for every block in the reserved buffer
+ * (BTA_WS_SOLVE or BTA_WS_SELINV; BTA_E_NORES without it).  Launched
+ * directly, never from a recorded graph. */
 int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
               double* rhs, int nrhs, int ldrhs, int trans, void* stream);
 
@@ -162,7 +203,7 @@ int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
  * SerinvSolver.selected_inversion (structured_solver.py:168-190).  With
  * BTA_OPT_BLAS on (the default) the Sigma_{i+1,i} products are rocBLAS dgemm
  * calls on `stream`; everything else runs in this library's kernels.  Returns
- * BTA_E_LIB if a rocBLAS call fails. */
+ * BTA_E_LIB if a rocBLAS call fails, BTA_E_NORES without BTA_WS_SELINV. */
 int bta_potri(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
               void* stream);
 

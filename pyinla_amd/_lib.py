@@ -17,9 +17,13 @@ _D = C.c_double
 # name -> (restype, argtypes); the authoritative list of exported symbols.
 SIGNATURES = {
     "bta_version": (_I, []),
-    "bta_ws_bytes": (C.c_size_t, [_I, _I, _I]),
+    "bta_ws_bytes": (C.c_size_t, [_I, _I, _I, _I]),
     "bta_ws_create": (_I, [_I, _I, _I, _I, _I, C.POINTER(_P)]),
     "bta_ws_destroy": (_I, [_P]),
+    "bta_ws_reserve": (_I, [_P, _I]),
+    "bta_ws_allocated": (C.c_size_t, [_P]),
+    "bta_ws_prepare": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I]),
+    "bta_logdet": (_I, [_P, _P, _P, _I, _I, _I, _P, _P]),
     "bta_potrf": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P]),
     "bta_ws_status": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(_D), C.POINTER(_I), _P]),
     "bta_ws_logdet_ptr": (_P, [_P]),
@@ -44,7 +48,11 @@ SIGNATURES = {
 }
 
 BTA_E = {-1: "invalid argument", -2: "shape does not match the workspace", -3: "in-kernel wait timed out",
-         -4: "rocBLAS call failed"}
+         -4: "rocBLAS call failed", -5: "the workspace lacks the reservation this call needs"}
+
+# workspace flags (include/pyinla_bta.h)
+WS_GRAPH, WS_SOLVE, WS_SELINV = 1, 2, 4
+RESERVE = {"factor": 0, "solve": WS_SOLVE, "selinv": WS_SOLVE | WS_SELINV}
 
 _lib = None
 

@@ -1,6 +1,7 @@
 // extern "C" entry points of libpyinla_bta.so (declared in include/pyinla_bta.h).
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -64,8 +65,18 @@ size_t ws_doubles(int n, int b, int a, size_t* work_out) {
 }
 
 // Run `enqueue` either directly on `st` or through a cached hipGraph.
+void drop_graphs(bta_ws* ws) {
+  if (!ws->graphs.empty()) {
+    hipDeviceSynchronize();  // a recorded graph may still be running
+    for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);
+    ws->graphs.clear();
+  }
+  ws->w.graphs_stale = false;
+}
+
 template <class F>
 int run_maybe_graph(bta_ws* ws, const GraphKey& key, hipStream_t st, F&& enqueue) {
+  if (ws->w.graphs_stale) drop_graphs(ws);  // a plan some graph recorded was freed
   if (!ws->use_graph) return enqueue(st);
   std::lock_guard<std::mutex> g(ws->mu);
   hipError_t e;
@@ -102,12 +113,50 @@ int check_shape(bta_ws* ws, int n, int b, int a) {
 
 extern "C" {
 
-int bta_version(void) { return 100; }
+int bta_version(void) { return 200; }
 
-size_t bta_ws_bytes(int n, int b, int a) { return 8 * ws_doubles(n, b, a, nullptr) + 4 * 8 + 64; }
+static size_t w_bytes(int n, int b) { return (size_t)n * b * b * sizeof(double); }
+static size_t g_bytes(int n, int b, int a) {
+  return ((size_t)n * b * b + (size_t)n * std::max(a, 1) * b) * sizeof(double);
+}
 
-int bta_ws_create(int device, int n, int b, int a, int use_graph, bta_ws_t* out) {
+size_t bta_ws_bytes(int n, int b, int a, int flags) {
+  if (n < 1 || b < 1 || a < 0) return 0;
+  size_t bytes = 8 * ws_doubles(n, b, a, nullptr) + ((size_t)n * ((b + 63) / 64) + 2) * 8 + 64;
+  if (b > 64) bytes += bta::FlowPlan::bytes_for((b + 63) / 64, (a + 63) / 64);
+  if (flags & (BTA_WS_SOLVE | BTA_WS_SELINV)) bytes += w_bytes(n, b);
+  if (flags & BTA_WS_SELINV) bytes += g_bytes(n, b, a);
+  return bytes;
+}
+
+int bta_ws_reserve(bta_ws_t ws, int flags) {
+  if (!ws || (flags & ~(BTA_WS_SOLVE | BTA_WS_SELINV))) return BTA_E_ARG;
+  bta::Workspace& w = ws->w;
+  hipError_t e = hipSetDevice(w.device);
+  if (e != hipSuccess) return (int)e;
+  if ((flags & (BTA_WS_SOLVE | BTA_WS_SELINV)) && !w.wfull) {
+    if ((e = hipMalloc(&w.wfull, w_bytes(w.n, w.b))) != hipSuccess) return (int)e;
+    w.w_valid = false;
+  }
+  if ((flags & BTA_WS_SELINV) && !w.gfull) {
+    if ((e = hipMalloc(&w.gfull, g_bytes(w.n, w.b, w.a))) != hipSuccess) return (int)e;
+  }
+  w.reserved |= flags;
+  return 0;
+}
+
+size_t bta_ws_allocated(bta_ws_t ws) {
+  if (!ws) return 0;
+  const bta::Workspace& w = ws->w;
+  size_t bytes = w.base_bytes;
+  if (w.wfull) bytes += w_bytes(w.n, w.b);
+  if (w.gfull) bytes += g_bytes(w.n, w.b, w.a);
+  return bytes + w.plan_bytes();
+}
+
+int bta_ws_create(int device, int n, int b, int a, int flags, bta_ws_t* out) {
   if (!out || n < 1 || b < 1 || a < 0 || a > 64) return BTA_E_ARG;
+  if (flags & ~(BTA_WS_GRAPH | BTA_WS_SOLVE | BTA_WS_SELINV)) return BTA_E_ARG;
   *out = nullptr;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return (int)e;
@@ -141,11 +190,22 @@ int bta_ws_create(int device, int n, int b, int a, int use_graph, bta_ws_t* out)
   w.epoch = u + nflags;
   w.info = u + nflags + 1;
   w.err = reinterpret_cast<int*>(u + nflags + 2);
-  ws->use_graph = use_graph;
-  if (use_graph) {
+  w.base_bytes = nd * sizeof(double) + (nflags + 2) * sizeof(unsigned long long) + 64;
+  int rc = 0;
+  if (bta::flow_wanted(w, b)) rc = bta::flow_alloc(w);
+  if (!rc && (flags & (BTA_WS_SOLVE | BTA_WS_SELINV))) rc = bta_ws_reserve(ws, flags & (BTA_WS_SOLVE | BTA_WS_SELINV));
+  if (rc) {
+    bta_ws_destroy(ws);
+    return rc;
+  }
+  ws->use_graph = (flags & BTA_WS_GRAPH) ? 1 : 0;
+  if (ws->use_graph) {
     hipStreamCreateWithFlags(&ws->gstream, hipStreamNonBlocking);
   }
-  if ((e = hipDeviceSynchronize()) != hipSuccess) return (int)e;
+  if ((e = hipDeviceSynchronize()) != hipSuccess) {
+    bta_ws_destroy(ws);
+    return (int)e;
+  }
   *out = ws;
   return 0;
 }
@@ -157,14 +217,12 @@ int bta_ws_destroy(bta_ws_t ws) {
   if (ws->w.tlog) bta_ws_set_timing(ws, 0);
   for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);
   if (ws->gstream) hipStreamDestroy(ws->gstream);
-  ws->w.trtri_plan.reset();
-  ws->w.potri_plan.reset();
-  ws->w.potri_plan_blas.reset();
+  ws->w.clear_plans();
+  ws->w.flow.reset();
   if (ws->w.blas) rocblas_destroy_handle(static_cast<rocblas_handle>(ws->w.blas));
-  ws->w.update_plan.reset();
-  hipFree(ws->w.winv);
+  if (ws->w.winv) hipFree(ws->w.winv);
   if (ws->w.gfull) hipFree(ws->w.gfull);
-  hipFree(ws->w.flags);
+  if (ws->w.flags) hipFree(ws->w.flags);
   if (ws->w.wfull) hipFree(ws->w.wfull);
   delete ws;
   return 0;
@@ -205,10 +263,38 @@ int bta_potrs(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, in
   hipStream_t st = as_stream(stream);
   rc = ensure_inverses(ws, D, n, b, st);
   if (rc) return rc;
-  GraphKey key{2, {(uintptr_t)D, (uintptr_t)L, (uintptr_t)B, (uintptr_t)T, (uintptr_t)rhs}, {n, b, a, nrhs, ldrhs, trans}};
-  return run_maybe_graph(ws, key, st, [&](hipStream_t s2) {
-    return bta::potrs_enqueue(ws->w, D, L, B, T, n, b, a, rhs, nrhs, ldrhs, trans, s2);
-  });
+  // the sweeps are a few GEMV launches per block: launched directly (a graph
+  // per right-hand-side buffer would grow without bound)
+  std::lock_guard<std::mutex> g(ws->mu);
+  return bta::potrs_enqueue(ws->w, D, L, B, T, n, b, a, rhs, nrhs, ldrhs, trans, st);
+}
+
+int bta_logdet(bta_ws_t ws, const double* D, const double* T, int n, int b, int a, double* out, void* stream) {
+  int rc = check_shape(ws, n, b, a);
+  if (rc) return rc;
+  if (!D || !out || (a > 0 && !T)) return BTA_E_ARG;
+  std::lock_guard<std::mutex> g(ws->mu);
+  return bta::logdet_enqueue(D, T, n, b, a, ws->w.logpart, out, as_stream(stream));
+}
+
+int bta_ws_prepare(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a, int ops) {
+  int rc = check_shape(ws, n, b, a);
+  if (rc) return rc;
+  if (!D || (n > 1 && !L) || (a > 0 && (!B || !T)) || (ops & ~7)) return BTA_E_ARG;
+  std::lock_guard<std::mutex> g(ws->mu);
+  if (ops & 1) {
+    rc = bta::potrf_prepare(ws->w, D, L, B, T, n, b, a);
+    if (rc) return rc;
+  }
+  if (ops & 2) {
+    rc = bta::trtri_prepare(ws->w, D, n, b);
+    if (rc) return rc;
+  }
+  if (ops & 4) {
+    rc = bta::potri_prepare(ws->w, D, L, B, T, n, b, a);
+    if (rc) return rc;
+  }
+  return 0;
 }
 
 int bta_block_inverses(bta_ws_t ws, double* D, int n, int b, void* stream) {
@@ -261,6 +347,8 @@ int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B
     if (w->w.n > 1 && !L[q]) return BTA_E_ARG;
     if (a[q] > 0 && (!B || !T || !B[q] || !T[q])) return BTA_E_ARG;
     if (w->w.device != ws[0]->w.device) return BTA_E_ARG;
+    for (int r = 0; r < q; ++r)
+      if (ws[r] == w) return BTA_E_ARG;  // one workspace per problem
     bta::ProbDesc d;
     d.D = D[q];
     d.L = L[q];
@@ -280,6 +368,7 @@ int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B
     d.a = a[q];
     d.nT = w->w.nT;
     d.nTa = (a[q] + 63) / 64;
+    d.fplan = w->w.flow.get();
     ps.push_back(d);
   }
   hipError_t e = hipSetDevice(ws[0]->w.device);
@@ -377,18 +466,20 @@ int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   std::lock_guard<std::mutex> g(ws->mu);
   if (option == BTA_OPT_TILE_UPDATE) ws->w.tile_update = value;
   else if (option == BTA_OPT_TILE_GROUPED) ws->w.tile_grouped = value;
-  else if (option == BTA_OPT_PANEL) ws->w.panel_mode = value;
+  else if (option == BTA_OPT_PANEL) {
+    ws->w.panel_mode = value;
+    if (bta::flow_wanted(ws->w, ws->w.b)) {
+      const int rc = bta::flow_alloc(ws->w);
+      if (rc) return rc;
+    }
+  }
   else if (option == BTA_OPT_FLOW_HELPERS) ws->w.flow_helpers = value;
   else if (option == BTA_OPT_FLOW_DELTA) ws->w.flow_delta = value;
   else if (option == BTA_OPT_BATCH_LANES) ws->w.batch_lanes = value;
-  else if (option == BTA_OPT_BLAS) {
-    ws->w.use_blas = value;
-    ws->w.potri_plan.reset();  // rebuilt with or without the dgemm calls
-  }
+  else if (option == BTA_OPT_BLAS) ws->w.use_blas = value;  // part of the plan key
   else return BTA_E_ARG;
   // recorded graphs embed the old launch shapes
-  for (auto& kv : ws->graphs) hipGraphExecDestroy(kv.second);
-  ws->graphs.clear();
+  drop_graphs(ws);
   return 0;
 }
 

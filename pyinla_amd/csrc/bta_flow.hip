@@ -5,7 +5,9 @@
 // diagonal Cholesky + inverse, TRSM of the sub-diagonal tile and the rank-64
 // update of the next diagonal tile back to back, while P helpers work through
 // their task lists.  Same algorithm and outputs as the per-column path
-// (serinv pobtaf's block loop, serinv/algs/pobtaf.py:108-170): L_ii in D_i,
+// (the published pobtaf block loop: POTRF of D_i, TRSM of L_i and B_i, then the
+// updates of D_{i+1}, B_{i+1} and T; called by the reference at
+// structured_solver.py:83-88): L_ii in D_i,
 // L_{i+1,i} in L_i, L_{a,i} in B_i, inverse of every diagonal tile in winv and
 // per-tile log-pivot sums in logpart.
 #include <algorithm>
@@ -227,7 +229,10 @@ static bool flow_build_list(const FlowShape& s, int helpers, FlowSchedule& out) 
       if (!diag && !(rho < nT && rho == c + 1)) {
         const int t = add(owner, FLOW_COST_TRSM, FLOW_TRSM, flow_encode(FLOW_TRSM, rho, c, c), tid);
         edge(prev, t);
-        edge(cf[c], t);
+        // inv(L_cc) is published when the chain solves the sub-diagonal tile
+        // of step c (after its waits for the step's hand-off tiles), or right
+        // after the factorization for the last column
+        edge(c + 1 < nT ? ct[c] : cf[c], t);
         xprod[tid] = t;
       }
     }
@@ -765,6 +770,11 @@ void FlowPlan::release() {
   nT = -1;
 }
 
+size_t FlowPlan::bytes_for(int nT, int nTa) {
+  const size_t R = 2 * (size_t)nT + nTa;
+  return ((R * nT + 2 * (size_t)nT) * 4096) * sizeof(double) + (R * nT + 3 * (size_t)nT) * 8;
+}
+
 int FlowPlan::build(int nT_, int nTa_) {
   release();
   nT = nT_;
@@ -777,22 +787,25 @@ int FlowPlan::build(int nT_, int nTa_) {
   const size_t nf = (size_t)R * nT + 3 * (size_t)nT;
   if ((e = hipMalloc(&flags, nf * sizeof(unsigned long long))) != hipSuccess) return (int)e;
   if ((e = hipMemset(flags, 0, nf * sizeof(unsigned long long))) != hipSuccess) return (int)e;
+  buf_bytes = bytes_for(nT, nTa);
   return 0;
 }
 
-int FlowPlan::sched(int helpers, int delta, const FlowSched** out) {
+int FlowPlan::sched(int nTa_s, int helpers, int delta, const FlowSched** out) {
+  if (nTa_s > nTa) return (int)hipErrorInvalidValue;
   for (const auto& sc : scheds)
-    if (sc.helpers == helpers && sc.delta == delta) {
+    if (sc.helpers == helpers && sc.delta == delta && sc.nTa == nTa_s) {
       *out = &sc;
       return 0;
     }
   FlowSched sc;
   sc.helpers = helpers;
   sc.delta = delta;
+  sc.nTa = nTa_s;
   std::vector<uint32_t> all;
   std::vector<int> off;
   for (int v = 0; v < 2; ++v) {  // 0: block with L, 1: last block (no L)
-    FlowShape s{nT, v == 0 ? nT : 0, nTa};
+    FlowShape s{nT, v == 0 ? nT : 0, nTa_s};
     FlowSchedule h;
     if (!flow_build_schedule(s, helpers, delta, h)) return -1;
     const int base = (int)all.size();
@@ -807,6 +820,7 @@ int FlowPlan::sched(int helpers, int delta, const FlowSched** out) {
     return (int)e;
   if ((e = hipMemcpy(sc.d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
     return (int)e;
+  sc.bytes = std::max<size_t>(1, all.size()) * sizeof(uint32_t) + off.size() * sizeof(int);
   scheds.push_back(sc);
   *out = &scheds.back();
   return 0;
@@ -852,7 +866,7 @@ FlowArgs flow_args(const FlowPlan& fp, const FlowSched& sc, double* D, double* L
   p.a = a;
   p.nT = nT;
   p.nTL = (i < n - 1) ? nT : 0;
-  p.nTa = (a > 0) ? fp.nTa : 0;
+  p.nTa = sc.nTa;
   p.blk = i + 1;
   p.helpers = sc.helpers;
   return p;

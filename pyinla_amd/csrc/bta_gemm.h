@@ -130,6 +130,7 @@ struct DevPlan {
   struct Entry { int job_off, pre_off, njobs, tiles, ts; };
   GemmJob* d_jobs = nullptr;
   int* d_prefix = nullptr;
+  size_t bytes = 0;  // device bytes of the job table and prefix sums
   std::vector<Entry> launches;
   DevPlan() = default;
   DevPlan(const DevPlan&) = delete;

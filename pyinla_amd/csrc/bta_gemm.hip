@@ -580,6 +580,7 @@ void DevPlan::release() {
   if (d_prefix) hipFree(d_prefix);
   d_jobs = nullptr;
   d_prefix = nullptr;
+  bytes = 0;
   launches.clear();
 }
 
@@ -606,6 +607,7 @@ int DevPlan::build(const std::vector<GemmLaunch>& ls) {
     return (int)e;
   if ((e = hipMemcpy(d_prefix, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
     return (int)e;
+  bytes = all.size() * sizeof(GemmJob) + pre.size() * sizeof(int);
   return 0;
 }
 

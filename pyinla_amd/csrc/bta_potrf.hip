@@ -392,6 +392,33 @@ __global__ void __launch_bounds__(BTA_THREADS) k_logdet_reduce_batched(const Pro
   if (threadIdx.x == 0) d.logdet[0] = s[0];
 }
 
+// 2 sum_j log X[j][j] of block blockIdx.x (the tip T for blockIdx.x == n),
+// strided per thread then a fixed tree: part[blockIdx.x].
+__global__ void __launch_bounds__(BTA_THREADS) k_block_logdiag(const double* D, const double* T, int n, int b, int a,
+                                                              double* part) {
+  __shared__ double s[BTA_THREADS];
+  const int i = blockIdx.x;
+  const double* X = i < n ? D + (int64_t)i * b * b : T;
+  const int m = i < n ? b : a;
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < m; j += BTA_THREADS) acc += log(X[(int64_t)j * m + j]);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = BTA_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[i] = 2.0 * s[0];
+}
+
+int logdet_enqueue(const double* D, const double* T, int n, int b, int a, double* part, double* out,
+                   hipStream_t st) {
+  const int np = n + (a > 0 ? 1 : 0);
+  hipLaunchKernelGGL(k_block_logdiag, dim3(np), dim3(BTA_THREADS), 0, st, D, T, n, b, a, part);
+  hipLaunchKernelGGL(k_logdet_reduce, dim3(1), dim3(BTA_THREADS), 0, st, part, np, out);
+  return (int)hipGetLastError();
+}
+
 // ------------------------------------------------------------ layout check
 
 __global__ void k_selftest_mfma(double* out) {
@@ -428,22 +455,28 @@ static bool use_flow(const Workspace& ws, int b) {
   return b > 64;  // a single tile column has no panel dataflow to overlap
 }
 
+bool flow_wanted(const Workspace& ws, int b) { return use_flow(ws, b); }
+
+int flow_alloc(Workspace& ws) {
+  if (ws.flow) return 0;
+  const int nT = (ws.b + 63) / 64, nTa = (ws.a + 63) / 64;
+  std::unique_ptr<FlowPlan> fp(new FlowPlan());
+  const int rc = fp->build(nT, nTa);
+  if (rc) return rc;
+  fp->spin_ticks = flow_spin_ticks(ws.device, 20.0);
+  ws.flow = std::move(fp);
+  return 0;
+}
+
 static int flow_prepare(Workspace& ws, int b, int a) {
   const int nT = (b + 63) / 64, nTa = (a + 63) / 64;
   int helpers = ws.flow_helpers > 0 ? ws.flow_helpers : flow_helpers_default(ws.device);
   if (helpers <= 0) return (int)hipErrorInvalidValue;
   helpers = std::min(helpers, 1023);
-  if (!(ws.flow && ws.flow->nT == nT && ws.flow->nTa == nTa)) {
-    ws.flow.reset(new FlowPlan());
-    ws.flow_sched = nullptr;
-    const int rc = ws.flow->build(nT, nTa);
-    if (rc) {
-      ws.flow.reset();
-      return rc;
-    }
-    ws.flow->spin_ticks = flow_spin_ticks(ws.device, 20.0);
-  }
-  return ws.flow->sched(helpers, ws.flow_delta, &ws.flow_sched);
+  // the exchange buffers come with the workspace (bta_ws_create, or the
+  // option call that selected the flow panel)
+  if (!ws.flow || ws.flow->nT != nT || ws.flow->nTa < nTa) return (int)hipErrorNotReady;
+  return ws.flow->sched(nTa, helpers, ws.flow_delta, &ws.flow_sched);
 }
 
 int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a) {
@@ -451,55 +484,44 @@ int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     const int rc = flow_prepare(ws, b, a);
     if (rc) return rc;
   }
-  if (choose_tile(ws.tile_update, b, b) != 128) {
-    ws.update_plan.reset();
-    return 0;
-  }
-  const void* key[4] = {D, L, B, T};
-  bool same = ws.update_plan && ws.update_a == a;
-  for (int q = 0; q < 4 && same; ++q) same = (ws.update_key[q] == key[q]);
-  if (same) return 0;
-  std::vector<GemmLaunch> ls;
-  for (int i = 0; i < n; ++i) {
-    GemmLaunch g(128);
-    double* Li = (i < n - 1) ? L + (int64_t)i * b * b : nullptr;
-    double* Bi = (a > 0) ? B + (int64_t)i * a * b : nullptr;
-    if (i < n - 1) {
-      GemmJob j = job(D + (int64_t)(i + 1) * b * b, b, b, b, 1.0, 1, 0);
-      add_seg(j, seg(Li, b, 0, Li, b, 1, b, -1.0));
-      g.add(j);
-      if (a > 0) {
-        GemmJob jb = job(B + (int64_t)(i + 1) * a * b, a, b, b, 1.0);
-        add_seg(jb, seg(Bi, b, 0, Li, b, 1, b, -1.0));
-        g.add(jb);
+  ws.update_cur = nullptr;
+  if (choose_tile(ws.tile_update, b, b) != 128) return 0;
+  const PlanKey key{{D, L, B, T}, a, 128, 0};
+  int rc = 0;
+  ws.update_cur = plan_lookup(ws, ws.update_plans, key, [&](DevPlan& plan) {
+    std::vector<GemmLaunch> ls;
+    for (int i = 0; i < n; ++i) {
+      GemmLaunch g(128);
+      double* Li = (i < n - 1) ? L + (int64_t)i * b * b : nullptr;
+      double* Bi = (a > 0) ? B + (int64_t)i * a * b : nullptr;
+      if (i < n - 1) {
+        GemmJob j = job(D + (int64_t)(i + 1) * b * b, b, b, b, 1.0, 1, 0);
+        add_seg(j, seg(Li, b, 0, Li, b, 1, b, -1.0));
+        g.add(j);
+        if (a > 0) {
+          GemmJob jb = job(B + (int64_t)(i + 1) * a * b, a, b, b, 1.0);
+          add_seg(jb, seg(Bi, b, 0, Li, b, 1, b, -1.0));
+          g.add(jb);
+        }
       }
+      if (a > 0) {
+        GemmJob jt = job(T, a, a, a, 1.0, 1, 0);
+        add_seg(jt, seg(Bi, b, 0, Bi, b, 1, b, -1.0));
+        g.add(jt);
+      }
+      ls.push_back(std::move(g));
     }
-    if (a > 0) {
-      GemmJob jt = job(T, a, a, a, 1.0, 1, 0);
-      add_seg(jt, seg(Bi, b, 0, Bi, b, 1, b, -1.0));
-      g.add(jt);
-    }
-    ls.push_back(std::move(g));
-  }
-  ws.update_plan.reset(new DevPlan());
-  const int rc = ws.update_plan->build(ls);
-  if (rc) {
-    ws.update_plan.reset();
-    return rc;
-  }
-  for (int q = 0; q < 4; ++q) ws.update_key[q] = key[q];
-  ws.update_a = a;
-  return 0;
+    return plan.build(ls);
+  }, &rc);
+  return rc;
 }
 
 int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a,
                   hipStream_t st) {
-  const void* key[4] = {D, L, B, T};
-  bool use_plan = ws.update_plan && ws.update_a == a;
-  for (int q = 0; q < 4 && use_plan; ++q) use_plan = (ws.update_key[q] == key[q]);
+  const bool use_plan = ws.update_cur != nullptr;
   const int nT = (b + 63) / 64;
   const int nTa = (a + 63) / 64;
-  const bool flow = use_flow(ws, b) && ws.flow && ws.flow->nT == nT && ws.flow->nTa == nTa && ws.flow_sched;
+  const bool flow = use_flow(ws, b) && ws.flow && ws.flow->nT == nT && ws.flow_sched && ws.flow_sched->nTa == nTa;
   if (use_flow(ws, b) && !flow) return (int)hipErrorNotReady;  // potrf_prepare not run for this shape
   hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, st, ws.epoch, ws.info, ws.err);
   for (int i = 0; i < n; ++i) {
@@ -543,7 +565,7 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     }
     if (a > 0) jobs.job[jobs.njobs++] = UpdJob{T, p.B, p.B, a, a, b, a, b, b, 1, 0, 0};
     TimedScope ts(ws, KC_UPDATE, st);
-    if (use_plan) ws.update_plan->launch(i, st);
+    if (use_plan) ws.update_cur->launch(i, st);
     else launch_gemm_jobs(jobs, st);
   }
   int npart = n * nT;
@@ -580,12 +602,22 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
     } else {
       int nmax = 0;
       for (const auto& d : ps) nmax = std::max(nmax, d.n);
+      // each problem uses its workspace's exchange buffers when it has them
+      // (problems run one after another within a lane, and every launch tags
+      // its flags with the call epoch, so no two launches share a live flag)
       fplans.clear();
+      owned.clear();
       for (const auto& d : ps) {
-        fplans.emplace_back(new FlowPlan());
-        const int rc = fplans.back()->build(d.nT, d.a > 0 ? d.nTa : 0);
+        const int nTa = d.a > 0 ? d.nTa : 0;
+        if (d.fplan && d.fplan->nT == d.nT && d.fplan->nTa >= nTa) {
+          fplans.push_back(d.fplan);
+          continue;
+        }
+        owned.emplace_back(new FlowPlan());
+        const int rc = owned.back()->build(d.nT, nTa);
         if (rc) return rc;
-        fplans.back()->spin_ticks = flow_spin_ticks(device, 20.0);
+        owned.back()->spin_ticks = flow_spin_ticks(device, 20.0);
+        fplans.push_back(owned.back().get());
       }
       // Problems run in waves of at most `lanes` (longest first), so that each
       // chain gets enough helpers; within a wave every launch covers the
@@ -609,7 +641,7 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
             const ProbDesc& d = ps[q];
             if (i >= d.n) continue;
             const FlowSched* sc = nullptr;
-            const int rc = fplans[q]->sched(helpers, delta, &sc);
+            const int rc = fplans[q]->sched(d.a > 0 ? d.nTa : 0, helpers, delta, &sc);
             if (rc) return rc;
             L.fb.e[L.fb.nent] = flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch,
                                           d.info, d.err, i, d.n, d.b, d.a);

@@ -28,6 +28,8 @@
 
 namespace bta {
 
+int potri_build(Workspace& ws, PotriPlan& pp, double* D, double* L, double* B, double* T, int n, int b, int a);
+
 __global__ void __launch_bounds__(BTA_THREADS) k_leaf_copy(double* W, const double* winv, int b, int nT) {
   const int blk = blockIdx.x / nT, k = blockIdx.x % nT;
   const int mv = min(64, b - k * 64);
@@ -38,54 +40,67 @@ __global__ void __launch_bounds__(BTA_THREADS) k_leaf_copy(double* W, const doub
     if (c < mv) dst[(int64_t)r * b + c] = src[r * 64 + c];
 }
 
+size_t Workspace::plan_bytes() const {
+  size_t t = 0;
+  for (const auto& kv : trtri_plans) t += kv.second->bytes;
+  for (const auto& kv : update_plans) t += kv.second->bytes;
+  for (const auto& kv : potri_plans) t += kv.second->grouped.bytes + kv.second->with_blas.bytes;
+  if (flow) {
+    t += flow->buf_bytes;
+    for (const auto& sc : flow->scheds) t += sc.bytes;
+  }
+  return t;
+}
+
+void Workspace::clear_plans() {
+  trtri_plans.clear();
+  update_plans.clear();
+  potri_plans.clear();
+  trtri_cur = update_cur = nullptr;
+  potri_cur = nullptr;
+}
+
 int trtri_prepare(Workspace& ws, const double* D, int n, int b) {
-  if (!ws.wfull) {
-    hipError_t e = hipMalloc(&ws.wfull, (size_t)n * b * b * sizeof(double));
-    if (e != hipSuccess) return (int)e;
-  }
-  if (ws.trtri_plan && ws.trtri_key == D && ws.trtri_tile == ws.tile_grouped) return 0;
-  const int nT = (b + 63) / 64;
-  std::vector<GemmLaunch> ls;
-  for (int s = 1; s < nT; s *= 2) {
-    const int ts = choose_tile(ws.tile_grouped, s * 64, s * 64);
-    GemmLaunch la(ts), lb(ts);
-    for (int t0 = 0; t0 + s < nT; t0 += 2 * s) {
-      const int r1 = t0 * 64, r2 = (t0 + s) * 64;
-      const int m1 = s * 64;
-      const int m2 = min((t0 + 2 * s) * 64, b) - r2;
-      for (int i = 0; i < n; ++i) {
-        double* W = ws.wfull + (int64_t)i * b * b;
-        const double* Di = D + (int64_t)i * b * b;
-        // T1^T = W11^T L21^T  -> upper block (r1, r2), m1 x m2
-        GemmJob ja = job(W + (int64_t)r1 * b + r2, m1, m2, b, 0.0);
-        add_seg(ja, seg(W + (int64_t)r1 * b + r1, b, 1, Di + (int64_t)r2 * b + r1, b, 1, m1, 1.0, TRI_A_UPPER));
-        la.add(ja);
-        // W21 = -W22 T1  -> lower block (r2, r1), m2 x m1
-        GemmJob jb = job(W + (int64_t)r2 * b + r1, m2, m1, b, 0.0);
-        add_seg(jb, seg(W + (int64_t)r2 * b + r2, b, 0, W + (int64_t)r1 * b + r2, b, 1, m2, -1.0, TRI_A_LOWER));
-        lb.add(jb);
+  if (!ws.wfull) return BTA_E_NORES;  // W is reserved with the workspace (bta_ws_reserve)
+  const PlanKey key{{D, nullptr, nullptr, nullptr}, 0, ws.tile_grouped, 0};
+  int rc = 0;
+  ws.trtri_cur = plan_lookup(ws, ws.trtri_plans, key, [&](DevPlan& plan) {
+    const int nT = (b + 63) / 64;
+    std::vector<GemmLaunch> ls;
+    for (int s = 1; s < nT; s *= 2) {
+      const int ts = choose_tile(ws.tile_grouped, s * 64, s * 64);
+      GemmLaunch la(ts), lb(ts);
+      for (int t0 = 0; t0 + s < nT; t0 += 2 * s) {
+        const int r1 = t0 * 64, r2 = (t0 + s) * 64;
+        const int m1 = s * 64;
+        const int m2 = min((t0 + 2 * s) * 64, b) - r2;
+        for (int i = 0; i < n; ++i) {
+          double* W = ws.wfull + (int64_t)i * b * b;
+          const double* Di = D + (int64_t)i * b * b;
+          // T1^T = W11^T L21^T  -> upper block (r1, r2), m1 x m2
+          GemmJob ja = job(W + (int64_t)r1 * b + r2, m1, m2, b, 0.0);
+          add_seg(ja, seg(W + (int64_t)r1 * b + r1, b, 1, Di + (int64_t)r2 * b + r1, b, 1, m1, 1.0, TRI_A_UPPER));
+          la.add(ja);
+          // W21 = -W22 T1  -> lower block (r2, r1), m2 x m1
+          GemmJob jb = job(W + (int64_t)r2 * b + r1, m2, m1, b, 0.0);
+          add_seg(jb, seg(W + (int64_t)r2 * b + r2, b, 0, W + (int64_t)r1 * b + r2, b, 1, m2, -1.0, TRI_A_LOWER));
+          lb.add(jb);
+        }
       }
+      ls.push_back(std::move(la));
+      ls.push_back(std::move(lb));
     }
-    ls.push_back(std::move(la));
-    ls.push_back(std::move(lb));
-  }
-  ws.trtri_plan.reset(new DevPlan());
-  const int rc = ws.trtri_plan->build(ls);
-  if (rc) {
-    ws.trtri_plan.reset();
-    return rc;
-  }
-  ws.trtri_key = D;
-  ws.trtri_tile = ws.tile_grouped;
-  return 0;
+    return plan.build(ls);
+  }, &rc);
+  return rc;
 }
 
 int trtri_enqueue(Workspace& ws, const double* D, int n, int b, hipStream_t st) {
   const int nT = (b + 63) / 64;
   hipLaunchKernelGGL(k_leaf_copy, dim3(n * nT), dim3(BTA_THREADS), 0, st, ws.wfull, ws.winv, b, nT);
-  for (size_t q = 0; q < ws.trtri_plan->launches.size(); ++q) {
+  for (size_t q = 0; q < ws.trtri_cur->launches.size(); ++q) {
     TimedScope ts(ws, KC_GROUPED, st);
-    ws.trtri_plan->launch((int)q, st);
+    ws.trtri_cur->launch((int)q, st);
   }
   return (int)hipGetLastError();
 }
@@ -93,17 +108,20 @@ int trtri_enqueue(Workspace& ws, const double* D, int n, int b, hipStream_t st) 
 int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a) {
   int rc = trtri_prepare(ws, D, n, b);
   if (rc) return rc;
-  const void* key[4] = {D, L, B, T};
-  bool same = ws.potri_plan && ws.potri_a == a && ws.potri_tile == ws.tile_grouped;
-  for (int q = 0; q < 4 && same; ++q) same = (ws.potri_key[q] == key[q]);
-  if (same) return 0;
   // G_i = L_i W_i and Ga_i = B_i W_i for every block (they depend on the
-  // factor only), kept for the whole recurrence
-  if (!ws.gfull) {
-    const size_t nd = (size_t)n * b * b + (size_t)n * std::max(a, 1) * b;
-    hipError_t e = hipMalloc(&ws.gfull, nd * sizeof(double));
-    if (e != hipSuccess) return (int)e;
-  }
+  // factor only), kept for the whole recurrence in the reserved buffer
+  // (n b^2 + n max(ws.a, 1) b doubles; a <= ws.a)
+  if (!ws.gfull) return BTA_E_NORES;
+  if (a > ws.a) return BTA_E_SHAPE;
+  const PlanKey key{{D, L, B, T}, a, ws.tile_grouped, ws.use_blas};
+  ws.potri_cur = plan_lookup(ws, ws.potri_plans, key, [&](PotriPlan& pp) {
+    return potri_build(ws, pp, D, L, B, T, n, b, a);
+  }, &rc);
+  return rc;
+}
+
+int potri_build(Workspace& ws, PotriPlan& pp, double* D, double* L, double* B, double* T, int n, int b, int a) {
+  int rc = 0;
   double* Gall = ws.gfull;
   double* Gaall = ws.gfull + (size_t)n * b * b;
   std::vector<GemmLaunch> ls;
@@ -181,15 +199,8 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     if (l2.tiles() > 0) ls.push_back(std::move(l2));
     if (l3.tiles() > 0) ls.push_back(std::move(l3));
   }
-  ws.potri_plan.reset(new DevPlan());
-  rc = ws.potri_plan->build(ls);
-  if (rc) {
-    ws.potri_plan.reset();
-    return rc;
-  }
-  ws.potri_plan_blas.reset();
-  ws.potri_blas.clear();
-  ws.potri_blas_order.clear();
+  rc = pp.grouped.build(ls);
+  if (rc) return rc;
   if (ws.use_blas && n > 1) {
     // The same recurrence with Sigma_{i+1,i} = -Sigma_{i+1,i+1} G_i as one
     // rocBLAS dgemm per block.  Its arrow term -Sigma_{n,i+1}^T Ga_i is written
@@ -202,7 +213,7 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     }
     std::vector<GemmLaunch> lb;
     lb.push_back(ls.front());  // l0
-    ws.potri_blas_order.push_back(0);
+    pp.order.push_back(0);
     for (int i = n - 1; i >= 0; --i) {
       const bool hasL = i < n - 1;
       double* Di = D + (int64_t)i * b * b;
@@ -212,12 +223,12 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
       if (a > 0 && !hasL) {
         GemmLaunch l2(ts);
         l2.add(arrow_job(i));
-        ws.potri_blas_order.push_back((int)lb.size());
+        pp.order.push_back((int)lb.size());
         lb.push_back(std::move(l2));
       }
       if (hasL) {
-        ws.potri_blas_order.push_back(-1 - (int)ws.potri_blas.size());
-        ws.potri_blas.push_back({D + (int64_t)(i + 1) * b * b, Gall + (int64_t)i * b * b, Li, a > 0 ? 1.0 : 0.0});
+        pp.order.push_back(-1 - (int)pp.gemms.size());
+        pp.gemms.push_back({D + (int64_t)(i + 1) * b * b, Gall + (int64_t)i * b * b, Li, a > 0 ? 1.0 : 0.0});
       }
       GemmLaunch l3(ts);
       if (a > 0 && i > 0) {
@@ -233,22 +244,14 @@ int potri_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
         l3.add(sd);
       }
       if (l3.tiles() > 0) {
-        ws.potri_blas_order.push_back((int)lb.size());
+        pp.order.push_back((int)lb.size());
         lb.push_back(std::move(l3));
       }
     }
-    ws.potri_plan_blas.reset(new DevPlan());
-    rc = ws.potri_plan_blas->build(lb);
-    if (rc) {
-      ws.potri_plan_blas.reset();
-      ws.potri_blas.clear();
-      ws.potri_blas_order.clear();
-      return rc;
-    }
+    rc = pp.with_blas.build(lb);
+    if (rc) return rc;
+    pp.has_blas = true;
   }
-  for (int q = 0; q < 4; ++q) ws.potri_key[q] = key[q];
-  ws.potri_a = a;
-  ws.potri_tile = ws.tile_grouped;
   return 0;
 }
 
@@ -263,23 +266,24 @@ int potri_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
   }
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cap) != hipSuccess) cap = hipStreamCaptureStatusNone;
-  if (!ws.potri_plan_blas || cap != hipStreamCaptureStatusNone) {
-    for (size_t q = 0; q < ws.potri_plan->launches.size(); ++q) {
+  PotriPlan& pp = *ws.potri_cur;
+  if (!pp.has_blas || cap != hipStreamCaptureStatusNone) {
+    for (size_t q = 0; q < pp.grouped.launches.size(); ++q) {
       TimedScope ts(ws, KC_GROUPED, st);
-      ws.potri_plan->launch((int)q, st);
+      pp.grouped.launch((int)q, st);
     }
     return (int)hipGetLastError();
   }
   rocblas_handle h = static_cast<rocblas_handle>(ws.blas);
   if (rocblas_set_stream(h, st) != rocblas_status_success) return BTA_E_LIB;
   const double minus_one = -1.0;
-  for (const int q : ws.potri_blas_order) {
+  for (const int q : pp.order) {
     if (q >= 0) {
       TimedScope ts(ws, KC_GROUPED, st);
-      ws.potri_plan_blas->launch(q, st);
+      pp.with_blas.launch(q, st);
       continue;
     }
-    const Workspace::BlasGemm& g = ws.potri_blas[-1 - q];
+    const PotriPlan::BlasGemm& g = pp.gemms[-1 - q];
     TimedScope ts(ws, KC_BLAS, st);
     // row-major C = A B is column-major C^T = B^T A^T
     if (rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, b, b, b, &minus_one, g.B, b, g.A, b,

@@ -11,6 +11,7 @@ tensors on the GPU; the arithmetic runs in libpyinla_bta.so.
 from __future__ import annotations
 
 import ctypes as C
+import hashlib
 import os
 import time
 
@@ -46,11 +47,16 @@ class SerinvSolver:
 
     Parameters follow structured_solver.py:22-29.  ``arrowhead_blocksize`` may
     be 0 (BT matrices; the arrow buffers are then ``None``).  ``use_graph``
-    replays each factorization / solve as a cached hipGraph.
+    replays each factorization / selected inversion as a cached hipGraph.
+    ``reserve`` sizes the device workspace once, as the reference allocates
+    its buffers once in ``__init__``: "selinv" (default; factor, solve and
+    selected inversion), "solve" (no selected inversion) or "factor" (no
+    solve either); an operation beyond the reservation raises.
     """
 
     def __init__(self, config=None, diagonal_blocksize: int = None, n_diag_blocks: int = None,
-                 arrowhead_blocksize: int = 0, device=None, use_graph: bool = True, **kwargs) -> None:
+                 arrowhead_blocksize: int = 0, device=None, use_graph: bool = True, reserve: str = "selinv",
+                 **kwargs) -> None:
         self.config = config if config is not None else SolverConfig()
         if diagonal_blocksize is None or n_diag_blocks is None:
             raise ValueError("diagonal_blocksize and n_diag_blocks are required")
@@ -69,11 +75,15 @@ class SerinvSolver:
         self.A_lower_diagonal_blocks = torch.empty((max(n - 1, 0), b, b), **f64)
         self.A_arrow_bottom_blocks = torch.empty((n, a, b), **f64) if a > 0 else None
         self.A_arrow_tip_block = torch.empty((a, a), **f64) if a > 0 else None
+        if reserve not in _lib.RESERVE:
+            raise ValueError(f"unknown reserve {reserve!r}: use 'factor', 'solve' or 'selinv'")
+        flags = (_lib.WS_GRAPH if use_graph else 0) | _lib.RESERVE[reserve]
         ws = C.c_void_p()
         with torch.cuda.device(self.device):
-            _lib.check(self.lib.bta_ws_create(self.device.index or 0, n, b, a, int(use_graph), C.byref(ws)),
+            _lib.check(self.lib.bta_ws_create(self.device.index or 0, n, b, a, flags, C.byref(ws)),
                        "bta_ws_create")
         self._ws = ws
+        self._ld_out = torch.zeros(1, **f64)
         self._scatter_cache = {}
         self.total_bytes = 0
         self.t_cholesky = 0.0
@@ -179,9 +189,16 @@ class SerinvSolver:
             raise np.linalg.LinAlgError(f"Matrix is not positive definite (first failing pivot at row {info - 1})")
 
     def logdet(self, sparsity: str) -> float:
-        """2 sum log diag of the factor (structured_solver.py:151-166)."""
+        """2 sum log diag of the blocks held now (structured_solver.py:151-166):
+        the diagonal blocks, plus the tip for "bta".  Like the reference it
+        reads the current blocks, whichever factorization left them there."""
         _check_sparsity(sparsity)
-        _, ld = self.status()
+        D, _, _, T, a = self._blocks(sparsity)
+        n, b = self.n_diag_blocks, self.diagonal_blocksize
+        with torch.cuda.device(self.device):
+            _lib.check(self.lib.bta_logdet(self._ws, _ptr(D), _ptr(T), n, b, a, _ptr(self._ld_out),
+                                           _stream_ptr(self.device)), "bta_logdet")
+        ld = float(self._ld_out.item())
         if np.isnan(ld):
             raise ValueError("Logdet is NaN. Check the input matrix.")
         return ld
@@ -245,15 +262,23 @@ class SerinvSolver:
                                           _stream_ptr(self.device)), "bta_potri")
         self._factored = None
 
-    def get_solver_memory(self) -> int:
-        """Bytes held by the block buffers (+ workspace); accumulates like the
-        reference does (structured_solver.py:540-550)."""
-        nbytes = sum(t.numel() * 8 for t in (self.A_diagonal_blocks, self.A_lower_diagonal_blocks,
-                                            self.A_arrow_bottom_blocks, self.A_arrow_tip_block)
+    def memory_bytes(self) -> int:
+        """Device bytes this solver holds now: the block buffers, the
+        workspace (its reservations and cached launch plans) and the cached
+        scatter maps."""
+        nbytes = sum(t.numel() * t.element_size() for t in (self.A_diagonal_blocks, self.A_lower_diagonal_blocks,
+                                                            self.A_arrow_bottom_blocks, self.A_arrow_tip_block)
                      if t is not None)
-        nbytes += int(self.lib.bta_ws_bytes(self.n_diag_blocks, self.diagonal_blocksize,
-                                            self.arrowhead_blocksize))
-        self.total_bytes += nbytes
+        nbytes += self._ld_out.numel() * 8
+        nbytes += int(self.lib.bta_ws_allocated(self._ws))
+        nbytes += sum(e[0].numel() * 4 + e[1].numel() * 8 for e in self._scatter_cache.values())
+        return nbytes
+
+    def get_solver_memory(self) -> int:
+        """Bytes held by the solver (``memory_bytes``), accumulated into
+        ``total_bytes`` on every call as the reference does
+        (structured_solver.py:540-550)."""
+        self.total_bytes += self.memory_bytes()
         return self.total_bytes
 
     # ------------------------------------------------- sparse <-> blocks
@@ -305,6 +330,15 @@ class SerinvSolver:
             codes[upper] = cd
         return idx, codes
 
+    @staticmethod
+    def _pattern_key(rows, cols) -> bytes:
+        """Digest of the full COO index arrays: two patterns share a scatter
+        map only if every (row, col) in COO order agrees."""
+        h = hashlib.blake2b(digest_size=20)
+        h.update(np.ascontiguousarray(rows, dtype=np.int64).tobytes())
+        h.update(np.ascontiguousarray(cols, dtype=np.int64).tobytes())
+        return h.digest()
+
     def _spmatrix_to_structured(self, A, sparsity: str) -> None:
         """Zero the blocks and scatter the sparse matrix into them
         (structured_solver.py:192-250).  ``A`` is a scipy sparse matrix or a
@@ -320,14 +354,13 @@ class SerinvSolver:
         else:
             coo = sps.coo_matrix(A)
             rows, cols, data = coo.row, coo.col, np.asarray(coo.data, dtype=np.float64)
-        key = (sparsity, len(data), rows[:4].tobytes(), cols[-4:].tobytes())
+        key = (sparsity, len(data), self._pattern_key(rows, cols))
         ent = self._scatter_cache.get(key)
         if ent is None:
             idx, codes = self._pattern_codes(rows, cols, sparsity, lower_only=False)
             ent = (torch.from_numpy(idx.astype(np.int32)).to(self.device),
                    torch.from_numpy(codes.view(np.int64)).to(self.device), len(idx))
             self._scatter_cache[key] = ent
-            self.total_bytes += ent[0].numel() * 4 + ent[1].numel() * 8
         src_idx, codes, count = ent
         for t in (self.A_diagonal_blocks, self.A_lower_diagonal_blocks, self.A_arrow_bottom_blocks,
                   self.A_arrow_tip_block):

@@ -332,18 +332,21 @@ class STGaussianModel:
         mat = np.asarray(mat, dtype=np.float64)
         npts = mat.shape[1]
         if wave is None:
-            # as many extra solver pairs (factor blocks and block inverses of
-            # both matrices) as fit in 70 % of the free device memory
+            # as many extra solver pairs (factor blocks of both matrices, the
+            # conditional's block inverses, with margin) as fit in 70 % of the
+            # free device memory
             free, _ = torch.cuda.mem_get_info(self.device)
-            per_pair = 8 * 2 * 3 * self.n * self.b * self.b
+            per_pair = 8 * 6 * self.n * self.b * self.b
             wave = 1 + int(0.7 * free // per_pair)
         wave = max(1, min(wave, npts))
         pool = getattr(self, "_fd_pool", None)
         if pool is None or len(pool[0]) != wave:
             self._fd_pool = None
             n, b, a = self.n, self.b, self.a
-            conds = [self.cond] + [SerinvSolver(None, b, n, a, device=self.device) for _ in range(wave - 1)]
-            priors = [self.prior] + [SerinvSolver(None, b, n, 0, device=self.device) for _ in range(wave - 1)]
+            conds = [self.cond] + [SerinvSolver(None, b, n, a, device=self.device, reserve="solve")
+                                   for _ in range(wave - 1)]
+            priors = [self.prior] + [SerinvSolver(None, b, n, 0, device=self.device, reserve="factor")
+                                     for _ in range(wave - 1)]
             batch = BatchedFactorization(conds + priors, ["bta"] * wave + ["bt"] * wave)
             self._fd_pool = (conds, priors, batch)
         conds, priors, batch = self._fd_pool

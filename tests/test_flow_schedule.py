@@ -108,34 +108,40 @@ class Sim:
         return Lf, np.linalg.inv(Lf)
 
     def chain(self):
+        """The chain workgroup with the device's wait and publish points
+        (flow_chain): per column k it waits for BOTH hand-off tiles before
+        the solve of (k+1, k); inv(L_kk) is published only then (stored one
+        step earlier), X(k+1, k) after the update of (k+1, k+1), and the last
+        inverse right after the last factorization."""
         nT = self.nT
         Lf, W = self.chol(self.tile(0, 0), self.valid(0))
         self.store("chain", 0, 0, Lf)
         self.winv[0] = W
-        self.flags.add(("w", 0))
+        if nT == 1:
+            self.flags.add(("w", 0))
         for k in range(nT - 1):
             if k == 0:
                 Cs = self.tile(1, 0).copy()
+                Cd = self.tile(1, 1).copy()
             else:
                 yield ("h", k, 0)
+                yield ("h", k, 1)
                 Cs = self.hacc[("s", k)].copy()
+                Cd = self.hacc[("d", k + 1)].copy()
             assert self.nupd.get((k + 1, k), 0) == k
+            assert self.nupd.get((k + 1, k + 1), 0) == k
             X = Cs @ W.T
             self.store("chain", k + 1, k, X)
             self.xbuf[(k + 1, k)] = X.copy()
-            self.flags.add(("x", k + 1, k))
-            if k == 0:
-                Cd = self.tile(1, 1).copy()
-            else:
-                yield ("h", k, 1)
-                Cd = self.hacc[("d", k + 1)].copy()
-            assert self.nupd.get((k + 1, k + 1), 0) == k
+            self.flags.add(("w", k))
             Cd -= X @ X.T
+            self.flags.add(("x", k + 1, k))
             self.nupd[(k + 1, k + 1)] = k + 1
             Lf, W = self.chol(Cd, self.valid(k + 1))
             self.store("chain", k + 1, k + 1, Lf)
             self.winv[k + 1] = W
-            self.flags.add(("w", k + 1))
+            if k + 2 == nT:
+                self.flags.add(("w", k + 1))
 
     def helper(self, h, tasks):
         nT = self.nT
@@ -310,14 +316,19 @@ def test_simulator_catches_planted_schedule_mistakes():
         Sim(D, L, B).run(bad, 0)
 
 
-@pytest.mark.parametrize("nT,nTL,nTa,helpers", [(32, 32, 1, 255), (32, 0, 0, 127), (16, 16, 1, 40)])
+@pytest.mark.parametrize("nT,nTL,nTa,helpers", [(32, 32, 1, 255), (32, 0, 0, 127), (16, 16, 1, 40),
+                                                 (32, 0, 1, 255), (71, 0, 0, 255), (71, 0, 1, 255),
+                                                 (71, 71, 1, 255), (71, 0, 0, 200), (40, 0, 0, 511)])
 def test_list_order_at_production_shape(nT, nTL, nTa, helpers):
-    """The list order (delta 0) for a b = 2048 panel and a full-device helper
-    count: no interleaving deadlocks and the factor is right (4-wide tiles keep
-    the numerics cheap; the schedule depends on the tile counts only)."""
-    ts = 4
+    """The list order (delta 0) for panels of b = 2048 and b = 4500 (71 tile
+    columns) with full-device helper counts, blocks with and without L: no
+    interleaving deadlocks and the factor is right (small tiles keep the
+    numerics cheap; the schedule depends on the tile counts only).  The last
+    block of the b = 4500 shape deadlocked on the device before TRSM tasks
+    depended on the chain step that publishes inv(L_kk)."""
+    ts = 4 if nT <= 32 else 2
     b = nT * ts
-    D, L, B = _panel(b, 3 if nTa else 0, nTL > 0, 5)
+    D, L, B = _panel(b, min(3, ts) if nTa else 0, nTL > 0, 5)
     lists = schedule(nT, nTL, nTa, helpers, 0)
     s = Sim(D, L, B, ts=ts)
     s.run(lists, 3)

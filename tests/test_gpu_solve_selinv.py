@@ -287,3 +287,116 @@ def test_block_inverses_built_ahead(cuda, use_graph):
     ob.pobtasi(D, L, B, T)
     assert _rel(s.A_diagonal_blocks.cpu().numpy(), D / 2.0) < RTOL
     assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L / 2.0) < RTOL
+
+
+def _oracle_all(kf, sparsity):
+    """Oracle factor, logdet, solve and selected inverse of kf's matrix."""
+    a = kf.a if sparsity == "bta" else 0
+    D, L = kf.Q_D.copy(), kf.Q_L.copy()
+    B = kf.Q_B.copy() if a else None
+    T = kf.Q_T.copy() if a else None
+    ob.pobtaf(D, L, B, T)
+    ld = ob.logdet(D, T)
+    n, b = kf.n, kf.b
+    rhs = np.random.default_rng(5).normal(size=n * b + a)
+    x = rhs.copy()
+    ob.pobtas(D, L, B, T, x, "N")
+    ob.pobtas(D, L, B, T, x, "C")
+    Ds, Ls = D.copy(), L.copy()
+    Bs = B.copy() if a else None
+    Ts = T.copy() if a else None
+    ob.pobtasi(Ds, Ls, Bs, Ts)
+    return dict(ld=ld, rhs=rhs, x=x, S=(Ds, Ls, Bs, Ts), F=(D, L, B, T))
+
+
+def _load(s, kf, sparsity):
+    a = kf.a if sparsity == "bta" else 0
+    s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if a else None, kf.Q_T if a else None)
+
+
+def _check_all(cuda, s, kf, sparsity, ref):
+    """factorize -> logdet -> solve -> selected_inversion on one solver."""
+    _load(s, kf, sparsity)
+    s.factorize(sparsity)
+    info, ld = s.status()
+    assert info == 0
+    assert abs(ld - ref["ld"]) <= 1e-10 * abs(ref["ld"])
+    x = torch.from_numpy(ref["rhs"].copy()).to(cuda)
+    s.solve(x, sparsity)
+    assert _rel(x.cpu().numpy(), ref["x"]) < RTOL
+    s.selected_inversion(sparsity)
+    Ds, Ls, Bs, Ts = ref["S"]
+    assert _rel(s.A_diagonal_blocks.cpu().numpy(), Ds) < RTOL
+    assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), Ls) < RTOL
+    if Bs is not None:
+        assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), Bs) < RTOL
+        assert _rel(s.A_arrow_tip_block.cpu().numpy(), Ts) < RTOL
+
+
+@pytest.mark.parametrize("b", [70, 130])
+def test_alternating_bta_and_bt_on_one_solver_with_graphs(cuda, b):
+    """The reference's evaluation loop factors Q_cond ("bta") then Q_prior
+    ("bt") on one solver, over and over (pyinla.py:655,1102), with
+    SerinvSolver's default graph replay: bta -> bt -> bta -> bt, each
+    followed by a solve and a selected inversion, against the oracle.  Every
+    launch plan a recorded graph uses must still be alive on the replays."""
+    n, a = 5, 5
+    kf = known_factor_bta(n, b, a, seed=41 + b)
+    refs = {sp: _oracle_all(kf, sp) for sp in ("bta", "bt")}
+    s = SerinvSolver(None, b, n, a, use_graph=True)
+    for _ in range(2):
+        for sp in ("bta", "bt"):
+            _check_all(cuda, s, kf, sp, refs[sp])
+
+
+def test_bt_selected_inversion_then_bta_on_bta_buffers(cuda):
+    """selected_inversion("bt") then a BTA factorization and
+    selected_inversion("bta") on the same solver: the G_a buffer of the
+    second call holds the full arrowhead (n a b doubles)."""
+    n, b, a = 4, 130, 7
+    kf = known_factor_bta(n, b, a, seed=77)
+    s = SerinvSolver(None, b, n, a, use_graph=False)
+    _check_all(cuda, s, kf, "bt", _oracle_all(kf, "bt"))
+    _check_all(cuda, s, kf, "bta", _oracle_all(kf, "bta"))
+
+
+def test_logdet_reads_the_held_blocks(cuda):
+    """logdet(sparsity) as the reference computes it
+    (structured_solver.py:151-166): 2 sum log diag of the blocks held now, so
+    after a BTA factorization logdet("bt") leaves the tip out."""
+    n, b, a = 4, 92, 4
+    kf = known_factor_bta(n, b, a, seed=3)
+    ref = _oracle_all(kf, "bta")
+    D, L, B, T = ref["F"]
+    s = SerinvSolver(None, b, n, a, use_graph=False)
+    _load(s, kf, "bta")
+    s.factorize("bta")
+    ld_bt = 2.0 * sum(np.log(np.diag(D[i])).sum() for i in range(n))
+    ld_tip = 2.0 * np.log(np.diag(T)).sum()
+    assert abs(s.logdet("bta") - ref["ld"]) <= 1e-12 * abs(ref["ld"])
+    assert abs(s.logdet("bt") - ld_bt) <= 1e-12 * abs(ld_bt)
+    assert abs(s.logdet("bta") - s.logdet("bt") - ld_tip) <= 1e-12 * abs(ref["ld"])
+
+
+def test_reservations_and_memory(cuda):
+    """A solver reserved for factorizations only refuses a solve (no hidden
+    allocation); the bytes it reports grow by exactly the reserved buffers."""
+    n, b, a = 3, 130, 4
+    kf = known_factor_bta(n, b, a, seed=12)
+    sf = SerinvSolver(None, b, n, a, use_graph=False, reserve="factor")
+    ss = SerinvSolver(None, b, n, a, use_graph=False, reserve="solve")
+    si = SerinvSolver(None, b, n, a, use_graph=False, reserve="selinv")
+    assert ss.memory_bytes() - sf.memory_bytes() == 8 * n * b * b
+    assert si.memory_bytes() - ss.memory_bytes() == 8 * (n * b * b + n * a * b)
+    _load(sf, kf, "bta")
+    sf.factorize("bta")
+    with pytest.raises(_lib.BTALibraryError, match="reservation"):
+        sf.solve(torch.zeros(n * b + a, dtype=torch.float64, device=cuda), "bta")
+    _load(ss, kf, "bta")
+    ss.factorize("bta")
+    ss.solve(torch.zeros(n * b + a, dtype=torch.float64, device=cuda), "bta")
+    with pytest.raises(_lib.BTALibraryError, match="reservation"):
+        ss.selected_inversion("bta")
+    # get_solver_memory accumulates like the reference (structured_solver.py:548)
+    m1 = si.get_solver_memory()
+    assert si.get_solver_memory() == 2 * m1

@@ -1,9 +1,12 @@
 """Pin the f(theta) oracle (oracle/model.py) on the CPU.
 
-f is checked against an independent dense float64 torch computation, and the
-reference-order central FD gradient against that computation's autograd
-gradient; the Gaussian-marginal identity f(x = 0) == f(x = x*) pins the
-quadratic term.  Coefficient tables are checked against hand-expanded values.
+f is pinned by the exact Gaussian marginal likelihood (f + log p(theta) +
+log N(y; 0, A Q_p^-1 A^T + D_o^-1) is a known theta-free constant, with the
+marginal density formed from a dense m x m covariance), its FD gradient by the
+autograd gradient of that marginal, and the structure of f by the INLA
+identity at several x.  It is also checked against a dense float64 torch
+restatement of the same formula.  Coefficient tables are checked against
+hand-expanded values.
 """
 import numpy as np
 import pytest
@@ -38,17 +41,144 @@ def test_coregional_f_and_fd_against_dense_autograd():
     assert np.abs(th.grad.numpy() - g).max() <= 1e-6 * np.abs(g).max()
 
 
-def test_gaussian_marginal_identity():
-    """log p(x|theta,y) evaluated at x* (what the reference does) and the same
-    posterior expressed at x = 0 must agree: -1/2 x*^T Q x* = -1/2 (0 - x*)^T Q (0 - x*)."""
+def _dense_model(p, table):
+    """Dense Q_prior (space-time part and fixed-effects tip), the stacked
+    observation matrix and the theta index of each observation row's noise
+    precision."""
+    Qp = om.q_prior(p, table).toarray()
+    obs = om._obs(p)
+    A = np.vstack([Av.toarray() for Av, _, _ in obs])
+    y = np.concatenate([np.asarray(yv) for _, yv, _ in obs])
+    idx = np.concatenate([np.full(Av.shape[0], i) for Av, _, i in obs])
+    return Qp, A, y, idx
+
+
+def _log_normal(x, mean, prec=None, cov=None):
+    """log N(x; mean, .) from a dense precision or covariance (all 2 pi terms)."""
+    d = x - mean
+    k = x.size
+    if prec is not None:
+        sign, ld = np.linalg.slogdet(prec)
+        return 0.5 * ld - 0.5 * d @ prec @ d - 0.5 * k * np.log(2 * np.pi)
+    sign, ld = np.linalg.slogdet(cov)
+    return -0.5 * ld - 0.5 * d @ np.linalg.solve(cov, d) - 0.5 * k * np.log(2 * np.pi)
+
+
+def _problems():
+    st = st_problem(nt=4, nx=4, ny=4, nb=3, seed=3)
+    co = coregional_problem(nv=2, nt=3, nx=4, ny=3, nb_per=2, obs_per_node=2, seed=5)
+    return [("st", st, CoefficientModel("baseline")), ("coregional", co, CoregionalCoefficients(2))]
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_f_is_the_exact_marginal_likelihood(case):
+    """For a Gaussian likelihood the INLA objective is exact: with every
+    constant kept,
+        f(theta) + log p(theta) + log N(y; 0, A Q_p^-1 A^T + D_o^-1)
+          = 1/2 nb ln(prec_fixed) - 1/2 m ln(2 pi),
+    the right side collecting the 2 pi terms f drops and the fixed-effects
+    tip the BT prior log-determinant leaves out (D_o = diag of the noise
+    precisions e^{theta_o} per observation row).  The marginal density is
+    formed from the m x m dense covariance, independently of any term of f,
+    and the identity must hold at several theta."""
+    name, p, cm = _problems()[case]
+    rng = np.random.default_rng(11)
+    const = 0.5 * p.nb * np.log(p.fixed_prec)
+
+    def residual(fval, th):
+        Qp, A, y, idx = _dense_model(p, cm.table(th))
+        Sy = A @ np.linalg.inv(Qp) @ A.T + np.diag(np.exp(-th[idx]))
+        logp = om.log_prior(th, p.prior_mean, p.prior_prec)
+        return fval + logp + _log_normal(y, np.zeros_like(y), cov=Sy) - (const - 0.5 * y.size * np.log(2 * np.pi))
+
+    for k in range(3):
+        th = np.asarray(p.theta) + (0 if k == 0 else rng.normal(0, 0.2, len(p.theta)))
+        f = om.evaluate_f(p, cm.table, th)
+        assert abs(residual(f, th)) <= 1e-10 * abs(f), (name, k)
+    # planted mistakes in the terms of f break the identity: the prior
+    # logdet with the fixed-effects tip, a flipped quadratic term, the
+    # likelihood's theta_o term dropped
+    th = np.asarray(p.theta)
+    pt = om.evaluate_f(p, cm.table, th, return_parts=True)
+    m = sum(Av.shape[0] for Av, _, _ in om._obs(p))
+    for bad in (pt["f"] - 0.5 * p.nb * np.log(p.fixed_prec), pt["f"] + pt["quad"],
+                pt["f"] + 0.5 * th[-1] * m / len(om._obs(p))):
+        assert abs(residual(bad, th)) > 1e-3 * abs(pt["f"]) or abs(residual(bad, th)) > 1.0
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_fd_gradient_is_the_gradient_of_the_marginal_likelihood(case):
+    """The reference-order central FD gradient of the oracle f equals the
+    autograd gradient of -log p(theta) - log N(y; 0, A Q_p(theta)^-1 A^T +
+    D_o(theta)^-1), built densely in torch from the coefficient table alone."""
+    name, p, cm = _problems()[case]
+    th0 = np.asarray(p.theta, dtype=np.float64)
+    _, g_fd = om.fd_gradient(p, cm.table, th0, eps=1e-4)
+    fn = cm.fn if hasattr(cm, "fn") else cm._fn
+    nv = getattr(p, "nv", 1)
+    obs = om._obs(p)
+    A = torch.as_tensor(np.vstack([Av.toarray() for Av, _, _ in obs]))
+    y = torch.as_tensor(np.concatenate([np.asarray(yv) for _, yv, _ in obs]))
+    idx = torch.as_tensor(np.concatenate([np.full(Av.shape[0], i) for Av, _, i in obs]))
+    Sd = [torch.as_tensor(s_.toarray()) for s_ in p.spatial]
+    Md = [torch.as_tensor(sps.csr_matrix(M).toarray()) for M in p.temporal]
+
+    def neg_log_marginal(t):
+        tab = fn(t[: len(th0) - len(obs) if nv > 1 else 3])
+        if tab.dim() == 2:
+            tab = tab[None, None]
+        n, b = p.nt, nv * p.ns
+        Qst = torch.zeros((n * b, n * b), dtype=torch.float64)
+        for v in range(nv):
+            for w in range(nv):
+                E = torch.zeros((nv, nv), dtype=torch.float64)
+                E[v, w] = 1.0
+                for m_, M in enumerate(Md):
+                    for k_, S in enumerate(Sd[: tab.shape[3]]):
+                        Qst = Qst + tab[v, w, m_, k_] * torch.kron(M, torch.kron(E, S))
+        Qp = torch.block_diag(Qst, p.fixed_prec * torch.eye(p.nb, dtype=torch.float64))
+        Sy = A @ torch.linalg.solve(Qp, A.T) + torch.diag(torch.exp(-t[idx]))
+        Ly = torch.linalg.cholesky(Sy)
+        z = torch.linalg.solve_triangular(Ly, y[:, None], upper=False)[:, 0]
+        logN = -torch.log(torch.diagonal(Ly)).sum() - 0.5 * z @ z
+        logp = torch.sum(-0.5 * torch.as_tensor(p.prior_prec) * (t - torch.as_tensor(p.prior_mean)) ** 2)
+        return -logp - logN
+
+    t = torch.tensor(th0, requires_grad=True)
+    neg_log_marginal(t).backward()
+    g = t.grad.numpy()
+    assert np.abs(g - g_fd).max() <= 1e-6 * np.abs(g).max(), (name, g, g_fd)
+
+
+def test_inla_identity_holds_at_every_x():
+    """log p(theta | y) up to theta-free constants is
+        log p(theta) + log p(x | theta) + log p(y | x, theta) - log p(x | theta, y)
+    at ANY x (the Gaussian conditional is exact).  The reference evaluates it
+    at x = 0 (pyinla.py:640-641); evaluated with full densities at x = 0,
+    x = x* and a random x it must give the same value, and that value is
+    -f(theta) up to the constant of test_f_is_the_exact_marginal_likelihood."""
     p = st_problem(nt=4, nx=4, ny=4, nb=2, seed=3)
-    parts = om.evaluate_f(p, CoefficientModel().table, p.theta, return_parts=True)
-    x = parts["x"]
-    table = CoefficientModel().table(p.theta)
-    Qc = om.q_cond(p, om.q_prior(p, table), p.theta)
-    rhs = np.exp(p.theta[3]) * (sps.csr_matrix(p.A).T @ p.y)
-    assert abs(parts["quad"] - float(x @ rhs)) <= 1e-10 * abs(parts["quad"])
-    assert np.linalg.norm(Qc @ x - rhs) <= 1e-10 * np.linalg.norm(rhs)
+    cm = CoefficientModel()
+    th = np.asarray(p.theta)
+    parts = om.evaluate_f(p, cm.table, th, return_parts=True)
+    Qp, A, y, idx = _dense_model(p, cm.table(th))
+    eo = np.exp(th[idx])
+    Qc = Qp + A.T @ (eo[:, None] * A)
+    xs = np.linalg.solve(Qc, A.T @ (eo * y))
+    assert np.abs(xs - parts["x"]).max() <= 1e-10 * np.abs(xs).max()
+    logp = om.log_prior(th, p.prior_mean, p.prior_prec)
+    vals = []
+    for x in (np.zeros_like(xs), xs, np.random.default_rng(2).normal(size=xs.size)):
+        lp_x = _log_normal(x, np.zeros_like(x), prec=Qp)
+        lp_y = np.sum(0.5 * np.log(eo) - 0.5 * eo * (y - A @ x) ** 2 - 0.5 * np.log(2 * np.pi))
+        lp_cond = _log_normal(x, xs, prec=Qc)
+        vals.append(logp + lp_x + lp_y - lp_cond)
+    assert max(vals) - min(vals) <= 1e-9 * abs(vals[0])
+    const = 0.5 * p.nb * np.log(p.fixed_prec) - 0.5 * y.size * np.log(2 * np.pi)
+    assert abs(-parts["f"] + const - vals[0]) <= 1e-10 * abs(parts["f"])
+    # the quadratic term equals x*^T b at x* = Q_c^-1 b
+    rhs = np.exp(th[3]) * (sps.csr_matrix(p.A).T @ p.y)
+    assert abs(parts["quad"] - float(xs @ rhs)) <= 1e-10 * abs(parts["quad"])
 
 
 def test_baseline_table_hand_expanded():
