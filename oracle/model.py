@@ -45,11 +45,17 @@ def q_st(spatial, temporal, table):
 
 
 def _obs(prob):
-    """[(A_v, y_v, theta index of its noise precision)] (one set for nv = 1)."""
+    """[(A_v, y_v, theta index of its noise precision)] (one set for nv = 1).
+    The indices are prob.noise_idx when the problem gives them, else the
+    last components of theta (one per observation set)."""
     if getattr(prob, "A_list", None) is not None:
-        d_q = len(prob.theta) - len(prob.A_list)
-        return [(sps.csr_matrix(A), y, d_q + v) for v, (A, y) in enumerate(zip(prob.A_list, prob.y_list))]
-    return [(sps.csr_matrix(prob.A), prob.y, 3)]
+        idx = getattr(prob, "noise_idx", None)
+        if idx is None:
+            d_q = len(prob.theta) - len(prob.A_list)
+            idx = [d_q + v for v in range(len(prob.A_list))]
+        return [(sps.csr_matrix(A), y, idx[v]) for v, (A, y) in enumerate(zip(prob.A_list, prob.y_list))]
+    idx = getattr(prob, "noise_idx", None)
+    return [(sps.csr_matrix(prob.A), prob.y, 3 if idx is None else idx[0])]
 
 
 def q_prior(prob, table):
@@ -87,6 +93,16 @@ def log_prior(theta, mean, prec):
     return float(np.sum(-0.5 * prec * (np.asarray(theta) - mean) ** 2))
 
 
+def _log_prior(prob, theta):
+    """log p(theta): the problem's prior objects when it has them (the PC and
+    Gaussian priors, pinned by tests/test_priors.py), else independent
+    Gaussians (prior_hyperparameters/gaussian.py:26-29)."""
+    pr = getattr(prob, "priors", None)
+    if pr is not None:
+        return float(pr.log_prior(theta))
+    return log_prior(theta, prob.prior_mean, prob.prior_prec)
+
+
 def evaluate_f(prob, table_fn, theta, return_parts=False):
     theta = np.asarray(theta, dtype=np.float64)
     n, b, a = prob.nt, getattr(prob, "nv", 1) * prob.ns, prob.nb
@@ -105,7 +121,7 @@ def evaluate_f(prob, table_fn, theta, return_parts=False):
     Dp, Lp, _, _ = scatter_bta(Qp, n, b, a, "bt")
     ob.pobtf(Dp, Lp)
     ld_p = ob.logdet(Dp)
-    logp = log_prior(theta, prob.prior_mean, prob.prior_prec)
+    logp = _log_prior(prob, theta)
     lik = sum(0.5 * theta[idx] * A.shape[0] - 0.5 * np.exp(theta[idx]) * float(y @ y) for A, y, idx in obs)
     f = (0.5 * ld_c - 0.5 * quad) - (logp + lik + 0.5 * ld_p)
     if return_parts:
@@ -214,7 +230,85 @@ def dense_f_torch(prob, table_fn_torch, theta):
     Lp = torch.linalg.cholesky(Qst)
     ld_p = 2 * torch.log(torch.diagonal(Lp)).sum()
     quad = x @ (Qc @ x)
-    mean = torch.as_tensor(prob.prior_mean)
-    prec = torch.as_tensor(prob.prior_prec)
-    logp = torch.sum(-0.5 * prec * (th - mean) ** 2)
+    logp = _log_prior_torch(prob, th)
     return (0.5 * ld_c - 0.5 * quad) - (logp + lik + 0.5 * ld_p)
+
+
+def _log_prior_torch(prob, th):
+    """log p(theta) in torch (differentiable): Gaussian, or the PC priors of
+    penalized_complexity.py:47-85 by hyperparameter type."""
+    import torch
+    pr = getattr(prob, "priors", None)
+    if pr is None:
+        mean = torch.as_tensor(prob.prior_mean)
+        prec = torch.as_tensor(prob.prior_prec)
+        return torch.sum(-0.5 * prec * (th - mean) ** 2)
+    total = torch.zeros((), dtype=torch.float64)
+    for i, p in enumerate(pr.priors):
+        t = th[i]
+        kind = getattr(p, "kind", "gaussian")
+        if kind == "gaussian":
+            total = total - 0.5 * p.precision * (t - p.mean) ** 2
+        elif kind == "r_s":
+            total = total + np.log(p.lam) - p.lam * torch.exp(-t) - t
+        elif kind == "r_t":
+            total = total + np.log(p.lam) - p.lam * torch.exp(-0.5 * t) + np.log(0.5) - 0.5 * t
+        elif kind in ("sigma_st", "sigma_e", "prec_o"):
+            total = total + np.log(p.lam) - p.lam * torch.exp(t) + t
+    return total
+
+
+def st_q_reference(spatial, temporal, r_s, r_t, sigma_st, gammas):
+    """One field's spatio-temporal precision as SpatioTemporalSubModel
+    builds it (spatio_temporal.py:114-149): with (gamma_s, gamma_t,
+    gamma_st) = gammas(r_s, r_t, sigma_st),
+      q1s = e^{2gs} c0 + g1, q2s = e^{4gs} c0 + 2 e^{2gs} g1 + g2,
+      q3s = e^{6gs} c0 + 3 e^{4gs} g1 + 3 e^{2gs} g2 + g3,
+      Q = e^{2gst} (m0 (x) q3s + e^{gt} m1 (x) q2s + e^{2gt} m2 (x) q1s)."""
+    c0, g1, g2, g3 = spatial[:4]
+    m0, m1, m2 = temporal
+    gs, gt, gst = (float(g) for g in gammas(r_s, r_t, sigma_st))
+    es, et, est = np.exp(gs), np.exp(gt), np.exp(gst)
+    q1s = es ** 2 * c0 + g1
+    q2s = es ** 4 * c0 + 2 * es ** 2 * g1 + g2
+    q3s = es ** 6 * c0 + 3 * es ** 4 * g1 + 3 * es ** 2 * g2 + g3
+    return (est ** 2 * (sps.kron(m0, q3s) + et * sps.kron(m1, q2s) + et ** 2 * sps.kron(m2, q1s))).tocsc()
+
+
+def coregional_q_prior_reference(prob, theta, sigma_st, gammas):
+    """Q_prior of CoregionalModel.construct_Q_prior (coregional_model.py:274-555)
+    restated block by block: per-field Qu_v, the sigma / lambda blocks
+    q_vw (:296-408), the field-major block matrix permuted to the
+    time-interleaved order [t][field][node] (:754-786), then the fixed-effects
+    tip.  theta layout: (r_s, r_t, prec_o) per field, sigmas, lambdas."""
+    nv, ns, nt = prob.nv, prob.ns, prob.nt
+    th = np.asarray(theta, dtype=np.float64)
+    Qu = [st_q_reference(prob.spatial, prob.temporal, th[3 * v], th[3 * v + 1], sigma_st[v], gammas)
+          for v in range(nv)]
+    s2 = [np.exp(th[3 * nv + v]) ** 2 for v in range(nv)]
+    lam = th[4 * nv:]
+    if nv == 2:
+        l01 = lam[0]
+        q = {(0, 0): Qu[0] / s2[0] + l01 ** 2 / s2[1] * Qu[1], (1, 0): -l01 / s2[1] * Qu[1],
+             (1, 1): Qu[1] / s2[1]}
+    else:
+        l01, l02, l12 = lam
+        q = {(0, 0): Qu[0] / s2[0] + l01 ** 2 / s2[1] * Qu[1] + l12 ** 2 / s2[2] * Qu[2],
+             (1, 0): -l01 / s2[1] * Qu[1] + l02 * l12 / s2[2] * Qu[2],
+             (2, 0): -l12 / s2[2] * Qu[2],
+             (1, 1): Qu[1] / s2[1] + l02 ** 2 / s2[2] * Qu[2],
+             (2, 1): -l02 / s2[2] * Qu[2],
+             (2, 2): Qu[2] / s2[2]}
+    blocks = [[None] * nv for _ in range(nv)]
+    for (v, w), m in q.items():
+        blocks[v][w] = m
+        if v != w:
+            blocks[w][v] = m.T
+    Qfm = sps.bmat(blocks, format="csr")  # field-major: index v*nt*ns + t*ns + s
+    n_re = nt * ns
+    perm = np.empty(nv * n_re, dtype=np.int64)  # interleaved position -> field-major index
+    for t in range(nt):
+        for v in range(nv):
+            perm[(t * nv + v) * ns:(t * nv + v + 1) * ns] = v * n_re + t * ns + np.arange(ns)
+    Qst = Qfm[perm][:, perm]
+    return sps.block_diag([Qst, prob.fixed_prec * sps.identity(prob.nb)], format="csr")

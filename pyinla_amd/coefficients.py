@@ -59,6 +59,52 @@ def _gamma(theta, manifold: str):
     return gamma_s, gamma_t, gamma_st
 
 
+def interpretable_to_model(r_s, r_t, sigma_st, manifold: str = "plane"):
+    """(gamma_s, gamma_t, gamma_st) of the model scale from the interpretable
+    (r_s, r_t, sigma_st): SpatioTemporalSubModel._interpretable2compute
+    (spatio_temporal.py:151-197), 2-D domain, alpha_s = 2, alpha_t = 1,
+    alpha_e = 1.  Accepts floats or torch scalars."""
+    th = torch.stack([torch.as_tensor(r_s, dtype=torch.float64), torch.as_tensor(r_t, dtype=torch.float64),
+                      torch.as_tensor(sigma_st, dtype=torch.float64)])
+    gs, gt, gst = _gamma(th, manifold)
+    return gs, gt, gst
+
+
+def model_to_interpretable(gamma_s, gamma_t, gamma_st, manifold: str = "plane", dim_spatial_domain: int = 2):
+    """(r_s, r_t, sigma_st) from the model scale (gamma_s, gamma_t, gamma_st):
+    SpatioTemporalSubModel.convert_theta_from_model2interpret
+    (spatio_temporal.py:199-263), the inverse of interpretable_to_model:
+      r_s = ln(sqrt(8 nu_s) / e^{gs}),  r_t = ln(e^{gt} sqrt(8 nu_t) / e^{alpha_s gs}),
+      plane:  sigma_st = ln(sqrt(c1) / (e^{gst} sqrt(e^{gt}) e^{gs (alpha - d/2)})),
+              c1 = Gamma(nu_t) Gamma(nu_s) / (Gamma(alpha_t) Gamma(alpha) (4 pi)^{d/2} (4 pi)^{1/2})
+      sphere: sigma_st = ln(sqrt(c_rt c_s) / (e^{gst} sqrt(e^{gt}))),
+              c_rt = Gamma(alpha_t - 1/2) / (Gamma(alpha_t) sqrt(4 pi)),
+              c_s = sum_{k<50} (2k+1) / (4 pi (e^{2 gs} + k(k+1))^alpha)."""
+    if dim_spatial_domain != 2:
+        raise ValueError("Only 2D spatial domain is supported for now.")
+    alpha_t, alpha_s, alpha_e = 1, 2, 1
+    alpha = alpha_e + alpha_s * (alpha_t - 0.5)
+    nu_s = alpha - 1
+    nu_t = alpha_t - 0.5
+    gs, gt, gst = float(gamma_s), float(gamma_t), float(gamma_st)
+    r_s = math.log(math.sqrt(8 * nu_s) / math.exp(gs))
+    r_t = math.log(math.exp(gt) * math.sqrt(8 * nu_t) / math.exp(gs) ** alpha_s)
+    if manifold == "sphere":
+        c_rt = math.gamma(alpha_t - 0.5) / (math.gamma(alpha_t) * math.sqrt(4 * math.pi))
+        c_s = 0.0
+        for k in range(50):
+            c_s += (2.0 * k + 1) / (4 * math.pi * (math.exp(gs) ** 2 + k * (k + 1)) ** alpha)
+        sigma_st = math.log(math.sqrt(c_rt * c_s) / (math.exp(gst) * math.sqrt(math.exp(gt))))
+    elif manifold == "plane":
+        c1_scale = (4 * math.pi) ** (dim_spatial_domain / 2.0) * (4 * math.pi) ** 0.5
+        c1 = math.gamma(nu_t) * math.gamma(nu_s) / (math.gamma(alpha_t) * math.gamma(alpha) * c1_scale)
+        sigma_st = math.log(math.sqrt(c1) / ((math.exp(gst) * math.sqrt(math.exp(gt)))
+                                             * math.exp(gs) ** (alpha - dim_spatial_domain / 2)))
+    else:
+        raise ValueError(f"Manifold not supported: {manifold}")
+    return r_s, r_t, sigma_st
+
+
 def reference_table(theta: torch.Tensor, manifold: str = "plane") -> torch.Tensor:
     """The reference parameterization (spatio_temporal.py:114-149):
 
@@ -94,6 +140,10 @@ class CoefficientModel:
             raise ValueError(kind)
         self.nM = 3
         self.d_st = 3
+
+    def noise_index(self, v: int = 0) -> int:
+        """theta index of the noise precision of observation set v."""
+        return self.d_st + v
 
     def table(self, theta) -> np.ndarray:
         th = torch.as_tensor(np.asarray(theta[: self.d_st], dtype=np.float64))
@@ -169,6 +219,9 @@ class CoregionalCoefficients:
         self.n_lam = nv * (nv - 1) // 2
         self.d_q = 3 * nv + self.n_lam
 
+    def noise_index(self, v: int) -> int:
+        return self.d_q + v
+
     def _fn(self, th: torch.Tensor) -> torch.Tensor:
         nv = self.nv
         tabs = torch.stack([baseline_table(th[3 * u:3 * u + 3]) for u in range(nv)])  # (u, m, k)
@@ -181,5 +234,60 @@ class CoregionalCoefficients:
 
     def jacobian(self, theta) -> np.ndarray:
         th = torch.as_tensor(np.asarray(theta[: self.d_q], dtype=np.float64))
+        J = torch.autograd.functional.jacobian(self._fn, th)  # (v, w, m, k, d)
+        return J.permute(4, 0, 1, 2, 3).detach().numpy().copy()
+
+
+class ReferenceCoregionalCoefficients:
+    """The reference's own coregional parameterization
+    (CoregionalModel.construct_Q_prior, coregional_model.py:274-408, with the
+    theta layout of coregional_model.py:96-150):
+
+      theta = (r_s, r_t, prec_o) per field v, then sigma_0 .. sigma_{nv-1},
+              then lambda_0_1 [, lambda_0_2, lambda_1_2];
+      Qu_v   = the reference ST precision of field v (reference_table) at
+               (r_s_v, r_t_v) and the field's fixed sigma_st (its sigma_st
+               hyperparameter is taken out of theta, :104-122);
+      Q_vw   = sum_u C[v][w][u](lambda) Qu_u / sigma_u^2, sigma_u = e^{theta_sigma_u},
+    with the weights of coregional_weights (q11 = Qu0/s0^2 + l01^2 Qu1/s1^2
+    + l12^2 Qu2/s2^2, ...).  Table (nv, nv, 3, 4) over (M0, M1, M2) x
+    (c0, g1, g2, g3); the Jacobian covers every theta component (zero rows
+    for the noise precisions)."""
+
+    def __init__(self, nv: int, manifold: str = "plane", sigma_st=0.0):
+        if nv not in (2, 3):
+            raise ValueError("coregional models have 2 or 3 fields")
+        self.nv = nv
+        self.manifold = manifold
+        self.sigma_st = np.broadcast_to(np.asarray(sigma_st, dtype=np.float64), (nv,)).copy()
+        self.nM, self.nK = 3, 4
+        self.n_lam = nv * (nv - 1) // 2
+        self.d = 3 * nv + nv + self.n_lam
+        self.d_q = self.d  # the table depends on theta up to here (noise rows are zero)
+
+    def noise_index(self, v: int) -> int:
+        return 3 * v + 2
+
+    def sigma_index(self, u: int) -> int:
+        return 3 * self.nv + u
+
+    def _fn(self, th: torch.Tensor) -> torch.Tensor:
+        nv = self.nv
+        tabs = []
+        for u in range(nv):
+            t_u = torch.stack([th[3 * u], th[3 * u + 1],
+                               torch.as_tensor(self.sigma_st[u], dtype=torch.float64)])
+            sig = torch.exp(th[3 * nv + u])
+            tabs.append(reference_table(t_u, self.manifold) / sig ** 2)
+        tabs = torch.stack(tabs)  # (u, m, k)
+        C = coregional_weights(th[4 * nv:4 * nv + self.n_lam], nv)  # (v, w, u)
+        return torch.einsum("vwu,umk->vwmk", C, tabs)
+
+    def table(self, theta) -> np.ndarray:
+        th = torch.as_tensor(np.asarray(theta[: self.d], dtype=np.float64))
+        return self._fn(th).detach().numpy().copy()
+
+    def jacobian(self, theta) -> np.ndarray:
+        th = torch.as_tensor(np.asarray(theta[: self.d], dtype=np.float64))
         J = torch.autograd.functional.jacobian(self._fn, th)  # (v, w, m, k, d)
         return J.permute(4, 0, 1, 2, 3).detach().numpy().copy()

@@ -33,7 +33,8 @@ import scipy.sparse as sps
 import torch
 
 from . import _lib
-from .coefficients import CoefficientModel, gaussian_log_prior, gaussian_log_prior_grad
+from .coefficients import CoefficientModel
+from .priors import priors_of
 from .solver import SerinvSolver, _ptr, _stream_ptr
 
 BUF_SHIFT = 60
@@ -185,11 +186,13 @@ class STGaussianModel:
         self.nM = nM
         # observation blocks, each with its noise precision index in theta
         n, b, a = self.n, self.b, self.a
+        noise = getattr(coef, "noise_index", lambda v: self.d_q + v)
         if getattr(prob, "A_list", None) is not None:
-            idxs = [self.d_q + v for v in range(len(prob.A_list))]
+            idxs = [noise(v) for v in range(len(prob.A_list))]
             self.obs = [_ObsSet(A, y, i, n, b, a, dev) for A, y, i in zip(prob.A_list, prob.y_list, idxs)]
         else:
-            self.obs = [_ObsSet(prob.A, prob.y, self.d_q, n, b, a, dev)]
+            self.obs = [_ObsSet(prob.A, prob.y, noise(0), n, b, a, dev)]
+        self.priors = priors_of(prob)
         self.m_obs = sum(o.m for o in self.obs)
         # solvers: conditional (BTA) and prior space-time part (BT)
         self.cond = SerinvSolver(None, b, nt, a, device=dev, use_graph=use_graph)
@@ -247,7 +250,7 @@ class STGaussianModel:
             self.rhs.add_(o.Aty, alpha=float(np.exp(theta[o.idx])))
 
     def _tail(self, theta, ld_c, ld_p, quad):
-        logp = gaussian_log_prior(theta, self.prob.prior_mean, self.prob.prior_prec)
+        logp = self.priors.log_prior(theta)
         lik = sum(0.5 * theta[o.idx] * o.m - 0.5 * np.exp(theta[o.idx]) * o.yty for o in self.obs)
         f = 0.5 * ld_c - 0.5 * quad - logp - lik - 0.5 * ld_p
         if np.isnan(f):
@@ -272,6 +275,7 @@ class STGaussianModel:
         block inverses (needed by its selected inversion) right behind its
         factorization, where they overlap the conditional solve."""
         theta = np.asarray(theta, dtype=np.float64)
+        self._saved = None  # the factors about to be formed belong to no backward pass yet
         cur = self._fork()
         with torch.cuda.stream(self.s_prior):
             self.assemble_prior(theta)
@@ -350,6 +354,7 @@ class STGaussianModel:
             batch = BatchedFactorization(conds + priors, ["bta"] * wave + ["bt"] * wave)
             self._fd_pool = (conds, priors, batch)
         conds, priors, batch = self._fd_pool
+        self._saved = None  # the pool's first pair is the model's own solvers
         fv = np.zeros(npts)
         order = list(range(npts)) if order is None else list(order)
         for w0 in range(0, npts, wave):
@@ -469,10 +474,32 @@ class STGaussianModel:
             "bta_bilinear_coo")
         return float(self.scalar[0])
 
-    def value_and_grad(self, theta):
-        """f(theta) and its adjoint gradient (one forward + selected inversions)."""
+    def forward_keep(self, theta) -> dict:
+        """The adjoint's forward pass: f(theta) with both factors kept in HBM
+        for the backward pass (and the prior's block inverses built behind
+        its factorization, where they overlap the conditional solve).
+        Returns the pieces of f; ``backward_grad`` consumes the saved state."""
         theta = np.asarray(theta, dtype=np.float64)
         parts = self.forward(theta, prior_inverses=True)
+        self._epoch = getattr(self, "_epoch", 0) + 1
+        self._saved = (self._epoch, theta.copy())
+        return parts
+
+    def backward_grad(self, token=None) -> np.ndarray:
+        """The adjoint's backward pass over the factors ``forward_keep`` left:
+        the selected inversion of each factor in place (pobtasi), the trace
+        and quadratic-form contractions, and the prior / likelihood terms.
+        ``token`` (the epoch forward_keep returned through ``saved_token``)
+        guards against factors overwritten by a later evaluation: if they no
+        longer belong to that forward, the forward is run again first."""
+        saved = getattr(self, "_saved", None)
+        if saved is None or (token is not None and token[0] != saved[0]):
+            if token is None:
+                raise RuntimeError("backward_grad needs a preceding forward_keep")
+            self.forward_keep(token[1])
+            saved = self._saved
+        theta = saved[1]
+        self._saved = None  # the selected inversions below overwrite the factors
         # the prior's selected inversion does not need x*: queued before the
         # host waits for the x* terms below
         cur = self._fork()
@@ -480,7 +507,7 @@ class STGaussianModel:
             self.prior.selected_inversion("bt")
             op = self._contract_launch(self.prior.A_diagonal_blocks, self.prior.A_lower_diagonal_blocks, None, 0,
                                        work=self.work_p, out=self.contract_out_p)
-        # terms that need x*, taken before the conditional factor is overwritten
+        # terms that need x* only
         Zq = self._contract(None, None, self.x, 1)
         xAtAx = [self._bilinear(o, 1, x=self.x) for o in self.obs]
         xAty = [float(torch.dot(self.x, o.Aty)) for o in self.obs]
@@ -500,9 +527,18 @@ class STGaussianModel:
         for o, tr, q, xb in zip(self.obs, trAtA, xAtAx, xAty):
             eo = float(np.exp(theta[o.idx]))
             g[o.idx] += 0.5 * eo * tr - eo * xb + 0.5 * eo * q - 0.5 * o.m + 0.5 * eo * o.yty
-        g -= gaussian_log_prior_grad(theta, self.prob.prior_mean, self.prob.prior_prec)
-        return parts["f"], g
+        g -= self.priors.grad(theta)
+        return g
 
+    @property
+    def saved_token(self):
+        """(epoch, theta) of the factors the last forward_keep left in HBM."""
+        return None if getattr(self, "_saved", None) is None else (self._saved[0], self._saved[1].copy())
+
+    def value_and_grad(self, theta):
+        """f(theta) and its adjoint gradient: forward_keep then backward_grad."""
+        parts = self.forward_keep(theta)
+        return parts["f"], self.backward_grad()
 
     def marginal_variances(self, theta, observations: bool = True):
         """Marginal variances of the latent parameters and of the
@@ -516,6 +552,7 @@ class STGaussianModel:
         Returns (latent variances, [observation variances per observation
         set]) as device tensors.  Overwrites the conditional solver's factor."""
         theta = np.asarray(theta, dtype=np.float64)
+        self._saved = None
         s = self.cond
         self.assemble_cond(theta)
         s.factorize("bta")
@@ -542,15 +579,22 @@ class STGaussianModel:
 
 
 class AdjointF(torch.autograd.Function):
-    """f(theta) as a differentiable torch scalar; backward uses the adjoint."""
+    """f(theta) as a differentiable torch scalar.
+
+    forward: assemble and factor Q_prior and Q_cond, solve for x*, evaluate
+    f; the factors stay in HBM.  backward: the selected inversion of both
+    factors in place (pobtasi) and the trace / quadratic contractions give
+    grad f; a forward that is never differentiated costs only the forward."""
 
     @staticmethod
     def forward(ctx, theta: torch.Tensor, model: STGaussianModel):
-        f, g = model.value_and_grad(theta.detach().cpu().numpy())
-        ctx.save_for_backward(torch.as_tensor(g, dtype=theta.dtype))
-        return torch.tensor(f, dtype=theta.dtype)
+        parts = model.forward_keep(theta.detach().cpu().numpy())
+        ctx.model = model
+        ctx.token = model.saved_token
+        ctx.theta_dtype = theta.dtype
+        return torch.tensor(parts["f"], dtype=theta.dtype)
 
     @staticmethod
     def backward(ctx, grad_out):
-        (g,) = ctx.saved_tensors
-        return grad_out * g, None
+        g = ctx.model.backward_grad(ctx.token)
+        return grad_out * torch.as_tensor(g, dtype=ctx.theta_dtype), None

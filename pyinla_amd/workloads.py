@@ -319,3 +319,16 @@ def coregional_problem(nv: int = 3, nt: int = 100, nx: int = 50, ny: int = 30, n
     return CoregionalProblem(nv=nv, ns=ns, nt=nt, nb=nb, spatial=spatial, temporal=temporal, A_list=A_list,
                              y_list=y_list, theta=theta, prior_mean=prior_mean,
                              prior_prec=np.full(theta.size, 0.5), fixed_prec=1e-3)
+
+
+def with_g3(prob):
+    """Append g3 = g1 c0^-1 g1 c0^-1 g1 to the spatial matrices (the fourth
+    matrix of the reference's spatio-temporal precision, q3s of
+    spatio_temporal.py:131-136); returns the problem."""
+    import scipy.sparse as sps
+    c0, g1, g2 = prob.spatial[:3]
+    ci = sps.diags(1.0 / c0.diagonal())
+    g3 = (g2 @ ci @ g1).tocsr()
+    g3.sort_indices()
+    prob.spatial = [c0, g1, g2, g3]
+    return prob

@@ -91,13 +91,93 @@ def test_adjoint_gradient_vs_dense_autograd_and_fd(cuda, small):
 
 
 def test_adjoint_autograd_function(cuda, small):
+    """AdjointF.forward runs only the forward (both factors stay in HBM, no
+    selected inversion); backward runs the selected inversions in place and
+    equals value_and_grad; a backward after another evaluation overwrote the
+    factors recomputes its forward first."""
     from pyinla_amd.st_model import AdjointF
     model = STGaussianModel(small, device=cuda, use_graph=True)
     th = torch.tensor(small.theta, dtype=torch.float64, requires_grad=True)
     f = AdjointF.apply(th, model)
+    ref = om.evaluate_f(small, model.coef.table, small.theta, return_parts=True)
+    assert abs(float(f) - ref["f"]) <= 1e-10 * max(abs(ref["ld_c"]), abs(ref["f"]))
+    # forward only: the conditional solver still holds the factor of Q_cond
+    assert abs(model.cond.logdet("bta") - ref["ld_c"]) <= 1e-10 * abs(ref["ld_c"])
     f.backward()
     _, g = model.value_and_grad(small.theta)
     np.testing.assert_allclose(th.grad.numpy(), g, rtol=1e-12, atol=0)
+    # stale factors: forward at theta, f at another theta, then backward
+    th2 = torch.tensor(small.theta, dtype=torch.float64, requires_grad=True)
+    f2 = AdjointF.apply(th2, model)
+    model.f(small.theta + 0.05)
+    f2.backward()
+    np.testing.assert_allclose(th2.grad.numpy(), g, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("nv", [2, 3])
+def test_reference_coregional_with_pc_priors(cuda, nv):
+    """The reference's own coregional parameterization (sigma_u per field,
+    coregional_model.py:274-408) with penalized-complexity priors on r_s,
+    r_t and the noise precisions, untyped PC priors on the sigmas (log p = 0
+    as in the reference) and Gaussian priors on the lambdas: f against the
+    oracle, the adjoint gradient against dense autograd and the GPU FD."""
+    from pyinla_amd.coefficients import ReferenceCoregionalCoefficients
+    from pyinla_amd.priors import GaussianPrior, HyperPriors, PenalizedComplexityPrior
+    from pyinla_amd.workloads import coregional_problem, with_g3
+    prob = with_g3(coregional_problem(nv=nv, nt=4, nx=6, ny=5, nb_per=2, obs_per_node=2, seed=9))
+    cm = ReferenceCoregionalCoefficients(nv, sigma_st=[0.0, 0.1, -0.1][:nv])
+    rng = np.random.default_rng(4)
+    th = np.zeros(cm.d)
+    for v in range(nv):
+        th[3 * v:3 * v + 3] = [0.3 + 0.1 * rng.normal(), 0.2 + 0.1 * rng.normal(), 1.0 + 0.1 * rng.normal()]
+    th[3 * nv:4 * nv] = 0.1 * rng.normal(size=nv)
+    th[4 * nv:] = 0.3 * rng.normal(size=cm.n_lam)
+    pri = []
+    for v in range(nv):
+        pri += [PenalizedComplexityPrior(0.01, 0.5, "r_s"), PenalizedComplexityPrior(0.01, 5.0, "r_t"),
+                PenalizedComplexityPrior(0.01, 5.0, "prec_o")]
+    pri += [PenalizedComplexityPrior(0.01, 1.0, None) for _ in range(nv)]
+    pri += [GaussianPrior(0.0, 2.0) for _ in range(cm.n_lam)]
+    prob.priors = HyperPriors(pri)
+    prob.theta = th
+    prob.noise_idx = [cm.noise_index(v) for v in range(nv)]
+    model = STGaussianModel(prob, coef=cm, device=cuda)
+    parts = model.forward(th)
+    ref = om.evaluate_f(prob, cm.table, th, return_parts=True)
+    assert abs(parts["f"] - ref["f"]) <= 1e-10 * max(abs(ref["ld_c"]), abs(ref["f"]))
+    assert abs(parts["logp"] - ref["logp"]) <= 1e-12 * max(1.0, abs(ref["logp"]))
+    f_adj, g_adj = model.value_and_grad(th)
+    tt = torch.tensor(th, dtype=torch.float64, requires_grad=True)
+    fd = om.dense_f_torch(prob, cm._fn, tt)
+    fd.backward()
+    assert abs(f_adj - float(fd.detach())) <= 1e-10 * abs(float(fd.detach()))
+    assert _rel(g_adj, tt.grad.numpy()) < 1e-6
+    _, g_fd = model.fd_gradient(th, eps=1e-4)
+    assert _rel(g_adj, g_fd) < 1e-5
+
+
+def test_run_pipeline(cuda, small):
+    """run() as PyINLA.run (pyinla.py:291-328): mode by L-BFGS-B with the
+    adjoint gradient, cov(theta) = inverse Hessian of f, marginal variances
+    at the mode; the mode matches the FD-gradient run and the variances the
+    oracle at the mode."""
+    from pyinla_amd.inla import run
+    cm = CoefficientModel("baseline")
+    model = STGaussianModel(small, device=cuda)
+    res = run(model, small.theta, gradient="adjoint", max_iter=8)
+    for key in ("theta", "x", "f", "grad_f", "f_values", "theta_values", "cov_theta",
+                "marginal_variances_latent", "marginal_variances_observations"):
+        assert key in res
+    th = res["theta"]
+    H = model.hessian_fd(th, eps=5e-3)
+    assert np.allclose(res["cov_theta"] @ H, np.eye(th.size), atol=1e-6)
+    lat_ref, obs_ref = om.marginal_variances(small, cm.table, th)
+    assert _rel(res["marginal_variances_latent"], lat_ref) < 1e-9
+    assert _rel(res["marginal_variances_observations"], np.concatenate(obs_ref)) < 1e-9
+    ref = om.evaluate_f(small, cm.table, th, return_parts=True)
+    assert _rel(res["x"], ref["x"]) < 1e-9
+    r_fd = run(STGaussianModel(small, device=cuda), small.theta, gradient="fd", max_iter=8)
+    assert np.abs(r_fd["theta"] - th).max() < 5e-3
 
 
 def test_dropin_sparse_interface(cuda, small):
