@@ -67,6 +67,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-nt", type=int, default=4, help="time steps of the bounded CPU sample")
     ap.add_argument("--graph", action="store_true", help="replay solver operations from hipGraphs")
+    ap.add_argument("--selinv-blas", type=int, default=None, choices=[0, 1],
+                    help="selected inversion: 1 = Sigma_{i+1,i} products through rocBLAS dgemm, 0 = every product "
+                         "in the library's grouped kernel (default: the library's default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (a known-size kernel to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters against)")
@@ -289,6 +292,9 @@ def main():
     n, b, a = prob.nt, prob.ns, prob.nb
     log(f"problem nt={n} ns={b} nb={a} m={prob.A.shape[0]} built in {time.perf_counter() - t0:.1f}s")
     model = STGaussianModel(prob, device=dev, use_graph=args.graph)
+    if args.selinv_blas is not None:
+        for s_ in (model.cond, model.prior):
+            s_.set_selinv_blas(bool(args.selinv_blas))
     theta = prob.theta.copy()
     log("model resident on GPU")
 

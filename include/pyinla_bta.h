@@ -162,16 +162,25 @@ int bta_logdet(bta_ws_t ws, const double* D, const double* T, int n, int b, int 
  * stream (not while that stream is under graph capture), 0 = the grouped
  * FP64-MFMA kernel for every product. */
 #define BTA_OPT_BLAS 7
+/* Persistent panel: the near-diagonal tiles ((c,c), (c+1,c), (c+2,c)), whose
+ * updates and solves the chain waits for, go to a pool of helpers / value
+ * helpers that own nothing else (default 8; 1..64). */
+#define BTA_OPT_FLOW_NEAR 8
+/* 64x64 grouped GEMM main loop: 0 (default) = two LDS buffers with loads
+ * issued two K chunks ahead, 1 = one LDS buffer (the earlier form, kept for
+ * comparison). */
+#define BTA_OPT_GEMM_KERNEL 9
 int bta_ws_set_option(bta_ws_t ws, int option, int value);
 
 /* Host-only diagnostic (no reference counterpart): the helper task lists of
  * the persistent panel launch for a panel of nT tile columns, nTL L tile rows
  * and nTa (0/1) arrow tile rows.  Task words are op << 30 | row << 20 |
  * col << 10 | k (op 0 = update with column k, 1 = solve, 2 = clear);
- * off[0..helpers] delimit the helpers' lists.  With tasks == off == NULL only
+ * off[0..helpers] delimit the helpers' lists; near_div is BTA_OPT_FLOW_NEAR
+ * (0 = the default).  With tasks == off == NULL only
  * *ntasks is returned.  Used by the CPU schedule simulator in the tests. */
-int bta_flow_schedule(int nT, int nTL, int nTa, int helpers, int delta, uint32_t* tasks, int64_t cap, int* off,
-                      int64_t* ntasks);
+int bta_flow_schedule(int nT, int nTL, int nTa, int helpers, int delta, int near_div, uint32_t* tasks, int64_t cap,
+                      int* off, int64_t* ntasks);
 
 /* Optional event timing (no reference counterpart; the reference times whole
  * calls with perf_counter, structured_solver.py:80-100).  While on, graphs are
@@ -282,6 +291,15 @@ int bta_bilinear_coo(const double* D, const double* L, const double* B, const do
 int bta_kron_contract(const double* D, const double* L, const double* x, int n, int b, int nv,
                       const int* rowptr, const int* colidx, const double* vals, int nnz, int nK,
                       int mode, double* work, double* out, void* stream);
+
+/* Diagnostic (no reference counterpart): C = beta C + alpha op(A) op(B) on row-
+ * major FP64 matrices (op(X) = X, or X^T when ta / tb = 1; C m x n, K = k)
+ * through the grouped-GEMM kernel the selected inversion uses: variant 0 =
+ * the pipelined 64x64 kernel, 1 = its single-buffered predecessor, 2 = 128x128
+ * tiles.  Builds a one-job plan (allocates), launches on `stream` and
+ * synchronises it; for tests and kernel measurements. */
+int bta_gemm(const double* A, int lda, int ta, const double* B, int ldb, int tb, double* C, int ldc, int m, int n,
+             int k, double alpha, double beta, int variant, void* stream);
 
 /* Diagnostic (no reference counterpart): runs one v_mfma_f64_16x16x4f64 with
  * A[i][k] = i + 100k, B[k][j] = j + 1000k and writes each lane's four

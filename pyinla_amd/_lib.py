@@ -44,7 +44,8 @@ SIGNATURES = {
     "bta_bilinear_coo": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, C.c_int64, _I, _P, _I, _P, _P]),
     "bta_quadform_rows": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, C.c_int64, _P, _P]),
     "bta_kron_contract": (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _P, _I, _I, _I, _P, _P, _P]),
-    "bta_flow_schedule": (_I, [_I, _I, _I, _I, _I, _P, C.c_int64, _P, C.POINTER(C.c_int64)]),
+    "bta_gemm": (_I, [_P, _I, _I, _P, _I, _I, _P, _I, _I, _I, _I, _D, _D, _I, _P]),
+    "bta_flow_schedule": (_I, [_I, _I, _I, _I, _I, _I, _P, C.c_int64, _P, C.POINTER(C.c_int64)]),
 }
 
 BTA_E = {-1: "invalid argument", -2: "shape does not match the workspace", -3: "in-kernel wait timed out",

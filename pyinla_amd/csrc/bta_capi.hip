@@ -376,7 +376,8 @@ int bta_batch_create(int nprob, bta_ws_t* ws, double** D, double** L, double** B
   bta_batch* bt = new bta_batch();
   bt->device = ws[0]->w.device;
   bt->ws.assign(ws, ws + nprob);
-  const int rc = bt->plan.build(ps, ws[0]->w.panel_mode, ws[0]->w.batch_lanes, ws[0]->w.flow_delta, ws[0]->w.device);
+  const int rc = bt->plan.build(ps, ws[0]->w.panel_mode, ws[0]->w.batch_lanes, ws[0]->w.flow_delta,
+                                ws[0]->w.flow_near, ws[0]->w.device);
   if (rc) {
     delete bt;
     return rc;
@@ -439,13 +440,15 @@ int bta_ws_status(bta_ws_t ws, int64_t* info, double* logdet, int* err, void* st
 
 double* bta_ws_logdet_ptr(bta_ws_t ws) { return ws ? ws->w.logdet : nullptr; }
 
-int bta_flow_schedule(int nT, int nTL, int nTa, int helpers, int delta, uint32_t* tasks, int64_t cap, int* off,
-                      int64_t* ntasks) {
+int bta_flow_schedule(int nT, int nTL, int nTa, int helpers, int delta, int near_div, uint32_t* tasks, int64_t cap,
+                      int* off, int64_t* ntasks) {
   if (nT < 1 || nT > 1023 || nTL < 0 || nTa < 0 || nTa > 1 || helpers < 1 || helpers > 1023 || delta < 0 ||
-      2 * nT + nTa > 1023)
+      2 * nT + nTa > 1023 || near_div < 0 || near_div > 64)
     return BTA_E_ARG;
   bta::FlowSchedule sc;
-  if (!bta::flow_build_schedule(bta::FlowShape{nT, nTL, nTa}, helpers, delta, sc)) return BTA_E_ARG;
+  bta::FlowShape shape{nT, nTL, nTa};
+  if (near_div > 0) shape.near_div = near_div;
+  if (!bta::flow_build_schedule(shape, helpers, delta, sc)) return BTA_E_ARG;
   if (ntasks) *ntasks = (int64_t)sc.tasks.size();
   if (!tasks && !off) return 0;
   if (!tasks || !off || cap < (int64_t)sc.tasks.size()) return BTA_E_ARG;
@@ -463,6 +466,8 @@ int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   if (option == BTA_OPT_FLOW_DELTA && (value < 0 || value > 64)) return BTA_E_ARG;
   if (option == BTA_OPT_BATCH_LANES && (value < 0 || value > 512)) return BTA_E_ARG;
   if (option == BTA_OPT_BLAS && value != 0 && value != 1) return BTA_E_ARG;
+  if (option == BTA_OPT_FLOW_NEAR && (value < 1 || value > 64)) return BTA_E_ARG;
+  if (option == BTA_OPT_GEMM_KERNEL && value != 0 && value != 1) return BTA_E_ARG;
   std::lock_guard<std::mutex> g(ws->mu);
   if (option == BTA_OPT_TILE_UPDATE) ws->w.tile_update = value;
   else if (option == BTA_OPT_TILE_GROUPED) ws->w.tile_grouped = value;
@@ -477,6 +482,8 @@ int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   else if (option == BTA_OPT_FLOW_DELTA) ws->w.flow_delta = value;
   else if (option == BTA_OPT_BATCH_LANES) ws->w.batch_lanes = value;
   else if (option == BTA_OPT_BLAS) ws->w.use_blas = value;  // part of the plan key
+  else if (option == BTA_OPT_FLOW_NEAR) ws->w.flow_near = value;
+  else if (option == BTA_OPT_GEMM_KERNEL) ws->w.gemm_variant = value;
   else return BTA_E_ARG;
   // recorded graphs embed the old launch shapes
   drop_graphs(ws);
@@ -525,6 +532,25 @@ int bta_ws_reset_timing(bta_ws_t ws) {
   hipDeviceSynchronize();
   for (int c = 0; c < bta::KC_COUNT; ++c) ws->w.tlog->used[c] = 0;
   return 0;
+}
+
+int bta_gemm(const double* A, int lda, int ta, const double* B, int ldb, int tb, double* C, int ldc, int m, int n,
+             int k, double alpha, double beta, int variant, void* stream) {
+  if (!A || !B || !C || m < 0 || n < 0 || k < 0 || (ta != 0 && ta != 1) || (tb != 0 && tb != 1)) return BTA_E_ARG;
+  if (variant < 0 || variant > 2) return BTA_E_ARG;
+  bta::GemmLaunch l(variant == 2 ? 128 : 64);
+  bta::GemmJob j = bta::job(C, m, n, ldc, beta);
+  bta::add_seg(j, bta::seg(A, lda, ta, B, ldb, tb, k, alpha));
+  l.add(j);
+  bta::DevPlan plan;
+  std::vector<bta::GemmLaunch> ls;
+  ls.push_back(std::move(l));
+  int rc = plan.build(ls);
+  if (rc) return rc;
+  hipStream_t st = as_stream(stream);
+  plan.launch(0, st, variant == 1 ? 1 : 0);
+  hipError_t e = hipStreamSynchronize(st);  // the plan is freed on return
+  return e == hipSuccess ? (int)hipGetLastError() : (int)e;
 }
 
 int bta_selftest_mfma(double* out, void* stream) {

@@ -79,6 +79,7 @@ __host__ __device__ inline void flow_decode(uint32_t w, int& op, int& rho, int& 
 // arrow tile (0, c) of B_{i+1}; rho == nT + 1 is the tip T (c = 0).
 struct FlowShape {
   int nT, nTL, nTa;
+  int near_div = 8;  // the near pool is helpers / near_div helpers (at most 3 nT)
   int rows() const { return nT + nTL + nTa; }
 };
 

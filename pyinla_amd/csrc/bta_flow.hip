@@ -31,8 +31,9 @@ namespace bta {
 // column's updates stays balanced; the next-block updates go to the same
 // pool.  Cleared upper tiles of D go by row-major index.
 static int flow_near_pool(const FlowShape& s, int helpers) {
+  const int div = s.near_div > 0 ? s.near_div : 8;
   if (helpers < 8) return 0;
-  return std::min(3 * s.nT, helpers / 8);
+  return std::min(3 * s.nT, helpers / div);
 }
 
 static bool flow_is_near(int nT, int rho, int c) { return rho < nT && rho >= c && rho - c <= 2; }
@@ -141,13 +142,13 @@ static void flow_build_levels(const FlowShape& s, int helpers, int delta, FlowSc
 // all workgroups resident the earliest unfinished task in that order can
 // always run, whatever the real durations: no interleaving can deadlock.
 #ifndef FLOW_COST_CHOL
-#define FLOW_COST_CHOL 45000  // chain: factor + inverse of a diagonal tile, stores
+#define FLOW_COST_CHOL 38600  // chain: factor + inverse of a diagonal tile, stores
 #endif
 #ifndef FLOW_COST_CTRSM
-#define FLOW_COST_CTRSM 8000  // chain: solve of the sub-diagonal tile, stores
+#define FLOW_COST_CTRSM 11400  // chain: solve of the sub-diagonal tile, stores, publish
 #endif
 #ifndef FLOW_COST_CUPD
-#define FLOW_COST_CUPD 8000  // chain: update of the next diagonal tile
+#define FLOW_COST_CUPD 5300  // chain: update of the next diagonal tile
 #endif
 #ifndef FLOW_COST_UPD
 #define FLOW_COST_UPD 12600  // rank-64 update, accumulator loaded and stored
@@ -791,10 +792,10 @@ int FlowPlan::build(int nT_, int nTa_) {
   return 0;
 }
 
-int FlowPlan::sched(int nTa_s, int helpers, int delta, const FlowSched** out) {
+int FlowPlan::sched(int nTa_s, int helpers, int delta, int near_div, const FlowSched** out) {
   if (nTa_s > nTa) return (int)hipErrorInvalidValue;
   for (const auto& sc : scheds)
-    if (sc.helpers == helpers && sc.delta == delta && sc.nTa == nTa_s) {
+    if (sc.helpers == helpers && sc.delta == delta && sc.nTa == nTa_s && sc.near_div == near_div) {
       *out = &sc;
       return 0;
     }
@@ -802,10 +803,11 @@ int FlowPlan::sched(int nTa_s, int helpers, int delta, const FlowSched** out) {
   sc.helpers = helpers;
   sc.delta = delta;
   sc.nTa = nTa_s;
+  sc.near_div = near_div;
   std::vector<uint32_t> all;
   std::vector<int> off;
   for (int v = 0; v < 2; ++v) {  // 0: block with L, 1: last block (no L)
-    FlowShape s{nT, v == 0 ? nT : 0, nTa_s};
+    FlowShape s{nT, v == 0 ? nT : 0, nTa_s, near_div};
     FlowSchedule h;
     if (!flow_build_schedule(s, helpers, delta, h)) return -1;
     const int base = (int)all.size();

@@ -138,7 +138,7 @@ struct DevPlan {
   ~DevPlan();
   void release();
   int build(const std::vector<GemmLaunch>& ls);
-  void launch(int idx, hipStream_t st) const;
+  void launch(int idx, hipStream_t st, int variant = 0) const;
   bool empty() const { return launches.empty(); }
 };
 

@@ -122,7 +122,7 @@ struct ChunkPrefetch {
   }
 };
 
-__global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(const GemmJob* __restrict__ jobs,
+__global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped_v1(const GemmJob* __restrict__ jobs,
                                                              const int* __restrict__ prefix, int njobs) {
   __shared__ double sA[64 * BTA_GEMM_LD];
   __shared__ double sB[64 * BTA_GEMM_LD];
@@ -207,6 +207,283 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
 #endif
       mma_nt_alpha<BTA_GEMM_LD>(acc, sA, sB, (kw + 3) & ~3, alpha);
     lds_barrier();  // not a wait for the next chunk's loads: its stores wait for them
+  }
+  // epilogue
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = acc_row(x, r), j = acc_col(y);
+        if (i >= rows || j >= cols) continue;
+        const double v = acc.c[x][y][r];
+        gdouble* cg = gview(ct);
+        if (g.lower && tr == tc) {
+          if (i >= j) {
+            cg[(int64_t)i * g.ldc + j] = v;
+            if (g.mirror && i > j) cg[(int64_t)j * g.ldc + i] = v;
+          }
+        } else {
+          cg[(int64_t)i * g.ldc + j] = v;
+          if (g.lower && g.mirror) gview(g.C)[(int64_t)(tc * 64 + j) * g.ldc + tr * 64 + i] = v;
+        }
+      }
+}
+
+// ------------------------------------------- 64x64 tiles, pipelined K loop
+//
+// The production form of the grouped GEMM.  K advances in chunks of 16; the
+// A chunk (64 rows of op(A) x 16) is kept in LDS as sA[i][k] (odd stride 17)
+// and the B chunk (16 x 64 columns of op(B)) as sB[k][j] (stride 68), so the
+// MFMA operand reads of a wave (16 rows x 4 k) hit distinct banks.  Two LDS
+// buffers and two register sets: while chunk t is multiplied out of one
+// buffer, chunk t+1 (loaded two chunks earlier) is stored to the other and
+// the loads of chunk t+3 are issued -- each global load has two chunks of
+// MFMA work to arrive.  Alpha is applied once, when the A chunk is stored to
+// LDS.  Interior chunks with even leading dimensions and 16-byte aligned
+// addresses load pairs of doubles; boundary chunks (ragged edges, partial K,
+// triangular masks) load element by element with zero fill.
+
+#define G2_KC 16
+#define G2_LDA (G2_KC + 1)
+#define G2_LDB (64 + 4)
+
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) f64x2 gf64x2;
+
+// Four elements of one 64 x 16 operand chunk per thread.  form 0 ("K
+// contiguous"): element (r, k) at g[r * ld + k], the thread owns rows
+// t/8 and t/8 + 32, columns 2 (t%8) and 2 (t%8) + 1; form 1 ("rows
+// contiguous"): element (r, k) at g[k * ld + r], the thread owns k = t/32
+// and t/32 + 8, rows 2 (t%32) and 2 (t%32) + 1.
+struct G2Op {
+  double r[4];
+  __device__ __forceinline__ void load(const double* __restrict__ g0, int64_t ld, int form, int rows, int kw, int mask,
+                                       int i0, int k0) {
+    const int t = threadIdx.x;
+    const gdouble* g = gview(g0);
+    mask = chunk_mask(mask, i0, rows, k0, kw);
+    const bool fast = mask == MASK_NONE && rows >= 64 && kw >= G2_KC && (ld & 1) == 0 &&
+                      (reinterpret_cast<uintptr_t>(g0) & 15) == 0;
+    if (form == 0) {
+      const int r0 = t >> 3, kk = 2 * (t & 7);
+      if (fast) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const f64x2 v = *(const gf64x2*)(g + (int64_t)(r0 + 32 * q) * ld + kk);
+          r[2 * q] = v.x;
+          r[2 * q + 1] = v.y;
+        }
+        return;
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int rr = r0 + 32 * q, k = kk + u;
+          r[2 * q + u] = (rr < rows && k < kw && !masked_out(mask, i0 + rr, k0 + k)) ? g[(int64_t)rr * ld + k] : 0.0;
+        }
+    } else {
+      const int k0t = t >> 5, rr = 2 * (t & 31);
+      if (fast) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const f64x2 v = *(const gf64x2*)(g + (int64_t)(k0t + 8 * q) * ld + rr);
+          r[2 * q] = v.x;
+          r[2 * q + 1] = v.y;
+        }
+        return;
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int k = k0t + 8 * q, ri = rr + u;
+          r[2 * q + u] = (ri < rows && k < kw && !masked_out(mask, i0 + ri, k0 + k)) ? g[(int64_t)k * ld + ri] : 0.0;
+        }
+    }
+  }
+  // A operand: sA[i][k] scaled by alpha
+  __device__ __forceinline__ void store_a(double* __restrict__ sA, int form, double alpha) const {
+    const int t = threadIdx.x;
+    if (form == 0) {
+      const int r0 = t >> 3, kk = 2 * (t & 7);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        sA[(r0 + 32 * q) * G2_LDA + kk] = alpha * r[2 * q];
+        sA[(r0 + 32 * q) * G2_LDA + kk + 1] = alpha * r[2 * q + 1];
+      }
+    } else {
+      const int k0t = t >> 5, rr = 2 * (t & 31);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        sA[rr * G2_LDA + k0t + 8 * q] = alpha * r[2 * q];
+        sA[(rr + 1) * G2_LDA + k0t + 8 * q] = alpha * r[2 * q + 1];
+      }
+    }
+  }
+  // B operand: sB[k][j]
+  __device__ __forceinline__ void store_b(double* __restrict__ sB, int form) const {
+    const int t = threadIdx.x;
+    if (form == 0) {
+      const int r0 = t >> 3, kk = 2 * (t & 7);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        sB[kk * G2_LDB + r0 + 32 * q] = r[2 * q];
+        sB[(kk + 1) * G2_LDB + r0 + 32 * q] = r[2 * q + 1];
+      }
+    } else {
+      const int k0t = t >> 5, rr = 2 * (t & 31);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) *(f64x2*)&sB[(k0t + 8 * q) * G2_LDB + rr] = f64x2{r[2 * q], r[2 * q + 1]};
+    }
+  }
+};
+
+// One staged chunk: both operands and the segment it belongs to.
+struct G2Stage {
+  G2Op a, b;
+  int seg;
+};
+
+__global__ void __launch_bounds__(BTA_THREADS, 4) k_gemm_grouped(const GemmJob* __restrict__ jobs,
+                                                                const int* __restrict__ prefix, int njobs) {
+  __shared__ double sA[2][64 * G2_LDA];
+  __shared__ double sB[2][G2_KC * G2_LDB];
+  const int jid = find_job(prefix, njobs, blockIdx.x);
+  const GemmJob& g = jobs[jid];
+  const int local = blockIdx.x - prefix[jid];
+  int tr, tc;
+  if (g.lower) decode_lower(local, tr, tc);
+  else {
+    const int tm = (g.m + 63) / 64, tn = (g.n + 63) / 64;
+    xcd_tile(local, prefix[jid], tm, tn, tr, tc);
+  }
+  const int rows = min(64, g.m - tr * 64), cols = min(64, g.n - tc * 64);
+  double* ct = g.C + (int64_t)(tr * 64) * g.ldc + tc * 64;
+  TileAcc acc;
+  if (g.beta != 0.0) {
+    acc_load(acc, ct, g.ldc, rows, cols);
+    if (g.beta != 1.0) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc.c[x][y] *= g.beta;
+    }
+  } else {
+    acc_zero(acc);
+  }
+  // K range of segment sid for this tile (triangular operands skip zero chunks)
+  auto seg_range = [&](int sid, int& kb, int& ke) {
+    const GemmSeg& sg = g.seg[sid];
+    kb = 0;
+    ke = sg.k;
+    if (sg.tri & TRI_A_LOWER) ke = min(ke, (tr + 1) * 64);
+    if (sg.tri & TRI_A_UPPER) kb = max(kb, tr * 64);
+    if (sg.tri & TRI_B_LOWER) kb = max(kb, tc * 64);
+    if (sg.tri & TRI_B_UPPER) ke = min(ke, (tc + 1) * 64);
+  };
+  // chunk cursor: (segment, k0, end of the segment's range); seg == nseg: done
+  struct Cur { int seg, k0, ke; };
+  auto settle = [&](Cur c) {  // move to the first non-empty chunk at or after c
+    while (c.seg < g.nseg && c.k0 >= c.ke) {
+      ++c.seg;
+      if (c.seg < g.nseg) seg_range(c.seg, c.k0, c.ke);
+    }
+    return c;
+  };
+  auto first = [&]() {
+    Cur c{0, 0, 0};
+    if (g.nseg > 0) seg_range(0, c.k0, c.ke);
+    return settle(c);
+  };
+  auto next = [&](Cur c) {
+    c.k0 += G2_KC;
+    return settle(c);
+  };
+  auto fetch = [&](const Cur& c, G2Stage& st) {
+    const GemmSeg& sg = g.seg[c.seg];
+    const int kw = min(G2_KC, c.ke - c.k0);
+    const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
+    const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
+    if (sg.ta == 0) st.a.load(sg.A + (int64_t)(tr * 64) * sg.lda + c.k0, sg.lda, 0, rows, kw, ma, tr * 64, c.k0);
+    else st.a.load(sg.A + (int64_t)c.k0 * sg.lda + tr * 64, sg.lda, 1, rows, kw, ma, tr * 64, c.k0);
+    if (sg.tb == 1) st.b.load(sg.B + (int64_t)(tc * 64) * sg.ldb + c.k0, sg.ldb, 0, cols, kw, mb, tc * 64, c.k0);
+    else st.b.load(sg.B + (int64_t)c.k0 * sg.ldb + tc * 64, sg.ldb, 1, cols, kw, mb, tc * 64, c.k0);
+    st.seg = c.seg;
+  };
+  auto stage = [&](const G2Stage& st, int buf) {
+    const GemmSeg& sg = g.seg[st.seg];
+    st.a.store_a(sA[buf], sg.ta, sg.alpha);
+    st.b.store_b(sB[buf], sg.tb == 1 ? 0 : 1);
+  };
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  auto mma = [&](int buf) {
+    const double* a0p = &sA[buf][(wm + li) * G2_LDA + lk];
+    const double* a1p = a0p + 16 * G2_LDA;
+    const double* b0p = &sB[buf][lk * G2_LDB + wn + li];
+    const double* b1p = b0p + 16;
+#pragma unroll
+    for (int k = 0; k < G2_KC; k += 4) {
+      const double a0 = a0p[k], a1 = a1p[k], b0 = b0p[k * G2_LDB], b1 = b1p[k * G2_LDB];
+      acc.c[0][0] = mfma_f64(a0, b0, acc.c[0][0]);
+      acc.c[0][1] = mfma_f64(a0, b1, acc.c[0][1]);
+      acc.c[1][0] = mfma_f64(a1, b0, acc.c[1][0]);
+      acc.c[1][1] = mfma_f64(a1, b1, acc.c[1][1]);
+    }
+  };
+  Cur c0 = first();
+  if (c0.seg < g.nseg) {
+    G2Stage s0, s1;
+    fetch(c0, s0);
+    Cur cf = next(c0);
+    bool have1 = cf.seg < g.nseg, have2 = false;
+    if (have1) fetch(cf, s1);
+    stage(s0, 0);
+    if (have1) {
+      const Cur c2 = next(cf);
+      have2 = c2.seg < g.nseg;
+      if (have2) {
+        fetch(c2, s0);
+        cf = c2;
+      }
+    }
+    __syncthreads();
+    // invariant at the top: the current chunk is in buffer 0, the next in s1
+    // (have1), the one after in s0 (have2); cf is the last fetched chunk
+    for (;;) {
+      mma(0);
+      if (!have1) break;
+      stage(s1, 1);
+      bool have3 = false;
+      if (have2) {
+        const Cur c3 = next(cf);
+        have3 = c3.seg < g.nseg;
+        if (have3) {
+          fetch(c3, s1);
+          cf = c3;
+        }
+      }
+      lds_barrier();
+      mma(1);
+      if (!have2) break;
+      stage(s0, 0);
+      bool have4 = false;
+      if (have3) {
+        const Cur c4 = next(cf);
+        have4 = c4.seg < g.nseg;
+        if (have4) {
+          fetch(c4, s0);
+          cf = c4;
+        }
+      }
+      lds_barrier();
+      have1 = have3;
+      have2 = have4;
+    }
   }
   // epilogue
 #pragma unroll
@@ -611,11 +888,14 @@ int DevPlan::build(const std::vector<GemmLaunch>& ls) {
   return 0;
 }
 
-void DevPlan::launch(int idx, hipStream_t st) const {
+void DevPlan::launch(int idx, hipStream_t st, int variant) const {
   const Entry& e = launches[idx];
   if (e.tiles <= 0) return;
   if (e.ts == BIG_TS)
     hipLaunchKernelGGL(k_gemm_grouped_big, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
+                       d_prefix + e.pre_off, e.njobs);
+  else if (variant == 1)
+    hipLaunchKernelGGL(k_gemm_grouped_v1, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
                        d_prefix + e.pre_off, e.njobs);
   else
     hipLaunchKernelGGL(k_gemm_grouped, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,

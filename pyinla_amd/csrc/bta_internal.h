@@ -41,7 +41,7 @@ struct FlowSched {
   uint32_t* d_tasks = nullptr;
   int* d_off = nullptr;
   int off_base[2] = {0, 0};
-  int helpers = 0, delta = 0, nTa = 0;
+  int helpers = 0, delta = 0, nTa = 0, near_div = 8;
   size_t bytes = 0;
 };
 
@@ -67,7 +67,7 @@ struct FlowPlan {
   int build(int nT, int nTa);
   // schedule for (nTa, helpers, delta) with nTa <= the buffers' nTa, built on
   // first request (host; never during capture)
-  int sched(int nTa, int helpers, int delta, const FlowSched** out);
+  int sched(int nTa, int helpers, int delta, int near_div, const FlowSched** out);
 };
 
 int flow_helpers_default(int device);
@@ -161,6 +161,8 @@ struct Workspace {
   int panel_mode = 0;
   int flow_helpers = 0;  // 0 = automatic
   int flow_delta = 0;    // 0: list-scheduled task order; >= 1: level order, far-tile update groups
+  int flow_near = 8;     // near pool = helpers / flow_near
+  int gemm_variant = 0;  // 64x64 grouped GEMM: 0 = pipelined K loop, 1 = single-buffered
   int batch_lanes = 0;   // problems per wave of a batched factorization (0 = 6)
   // persistent-panel exchange buffers, sized for the workspace's arrowhead
   // (allocated by bta_ws_create when the flow panel can run)
@@ -249,7 +251,7 @@ struct BatchPlan {
   ~BatchPlan();
   // lanes: problems factored concurrently in the persistent form (0 = 6,
   // at most FLOW_BATCH_MAX)
-  int build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int delta, int device);
+  int build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int delta, int near_div, int device);
   int enqueue(hipStream_t st) const;
 };
 

@@ -476,7 +476,7 @@ static int flow_prepare(Workspace& ws, int b, int a) {
   // the exchange buffers come with the workspace (bta_ws_create, or the
   // option call that selected the flow panel)
   if (!ws.flow || ws.flow->nT != nT || ws.flow->nTa < nTa) return (int)hipErrorNotReady;
-  return ws.flow->sched(nTa, helpers, ws.flow_delta, &ws.flow_sched);
+  return ws.flow->sched(nTa, helpers, ws.flow_delta, ws.flow_near, &ws.flow_sched);
 }
 
 int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a) {
@@ -583,7 +583,8 @@ BatchPlan::~BatchPlan() {
   if (d_ent) hipFree(d_ent);
 }
 
-int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int delta, int device) {
+int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes, int delta, int near_div,
+                     int device) {
   probs = ps;
   // Persistent dataflow form when every problem has more than one tile column
   // (or when asked for): each problem gets a chain and an equal share of the
@@ -641,7 +642,7 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
             const ProbDesc& d = ps[q];
             if (i >= d.n) continue;
             const FlowSched* sc = nullptr;
-            const int rc = fplans[q]->sched(d.a > 0 ? d.nTa : 0, helpers, delta, &sc);
+            const int rc = fplans[q]->sched(d.a > 0 ? d.nTa : 0, helpers, delta, near_div, &sc);
             if (rc) return rc;
             L.fb.e[L.fb.nent] = flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch,
                                           d.info, d.err, i, d.n, d.b, d.a);

@@ -100,7 +100,7 @@ int trtri_enqueue(Workspace& ws, const double* D, int n, int b, hipStream_t st) 
   hipLaunchKernelGGL(k_leaf_copy, dim3(n * nT), dim3(BTA_THREADS), 0, st, ws.wfull, ws.winv, b, nT);
   for (size_t q = 0; q < ws.trtri_cur->launches.size(); ++q) {
     TimedScope ts(ws, KC_GROUPED, st);
-    ws.trtri_cur->launch((int)q, st);
+    ws.trtri_cur->launch((int)q, st, ws.gemm_variant);
   }
   return (int)hipGetLastError();
 }
@@ -270,7 +270,7 @@ int potri_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
   if (!pp.has_blas || cap != hipStreamCaptureStatusNone) {
     for (size_t q = 0; q < pp.grouped.launches.size(); ++q) {
       TimedScope ts(ws, KC_GROUPED, st);
-      pp.grouped.launch((int)q, st);
+      pp.grouped.launch((int)q, st, ws.gemm_variant);
     }
     return (int)hipGetLastError();
   }
@@ -280,7 +280,7 @@ int potri_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
   for (const int q : pp.order) {
     if (q >= 0) {
       TimedScope ts(ws, KC_GROUPED, st);
-      pp.with_blas.launch(q, st);
+      pp.with_blas.launch(q, st, ws.gemm_variant);
       continue;
     }
     const PotriPlan::BlasGemm& g = pp.gemms[-1 - q];

@@ -165,8 +165,8 @@ __device__ __forceinline__ void panel_ldlt8(const double* __restrict__ s, int p,
 // On exit: s holds L (lower, zero strict upper, identity outside mv),
 //          w holds L^-1 (lower), both with stride BTA_LDS_LD.
 // scratch: LDS, >= 80 doubles.
-__device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, double* __restrict__ w,
-                                                         double* __restrict__ scratch, int mv) {
+__device__ __forceinline__ TileCholResult tile_chol_inv_p8(double* __restrict__ s, double* __restrict__ w,
+                                                            double* __restrict__ scratch, int mv) {
   const int tid = threadIdx.x;
   const int wv = tid >> 6, lane = tid & 63;
   double* ub = scratch;       // [8][8] unit lower factor of the panel's diagonal block
@@ -293,4 +293,190 @@ __device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, 
   }
   __syncthreads();
   return res;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked version: the 64x64 tile as 4 x 4 blocks of 16.  For each panel j,
+//   phase 1: wave 0 factors and inverts the 16x16 diagonal block A_jj in
+//            registers (lane c holds column c; the pivot column is broadcast
+//            with v_readlane, so no LDS round trip or barrier sits inside the
+//            16 pivots); meanwhile waves 1..3 form T_jk = sum_{m=k}^{j-1}
+//            L_jm E_mk for the inverse row j (k < j);
+//   phase 2: waves 1..3 form the panel's blocks below the diagonal,
+//            L_ij = A_ij E_jj^T (i > j), and the inverse blocks
+//            E_jk = -E_jj T_jk (k < j) -- always three 16x16 products;
+//   phase 3: all waves apply the trailing updates A_ik -= L_ij L_kj^T
+//            (j < k <= i).
+// Every off-diagonal product is a 16x16x16 FP64 MFMA product (4
+// v_mfma_f64_16x16x4f64) on LDS operands.  Three barriers per panel.
+// Inside the 16x16 factorization every column c of E = L^-1 receives the
+// same row operations as the matrix (E starts at I): row p is scaled by
+// 1 / L_pp and row i > p loses L_ip times row p.
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// acc +/-= X(rx.., kx..) * op(Y): NT form (ty == 0: Y rows ry.., K cols ky..),
+// or NN form (ty == 1: Y rows ky.. (K), cols ry..), K = 16, LDS stride BTA_LDS_LD.
+template <bool NEG, int TY>
+__device__ __forceinline__ f64x4 mma16x16(const double* __restrict__ X, int rx, int kx, const double* __restrict__ Y,
+                                          int ry, int ky, f64x4 acc) {
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lk = lane >> 4;
+  double a[4], b[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    a[k] = X[(rx + li) * BTA_LDS_LD + kx + 4 * k + lk];
+    b[k] = TY == 0 ? Y[(ry + li) * BTA_LDS_LD + ky + 4 * k + lk] : Y[(ky + 4 * k + lk) * BTA_LDS_LD + ry + li];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) acc = mfma_f64(NEG ? -a[k] : a[k], b[k], acc);
+  return acc;
+}
+
+// 16x16 Cholesky and inverse of the block at (r0, r0) of s (lower triangle
+// read, symmetric), by one wave; writes L (zero strict upper) to s and L^-1 to
+// w at the same block.  Lane c (of every 16-lane row) holds column c of the
+// symmetric matrix and of E.  At pivot p the finished columns c <= p take a
+// zero multiplier (their registers keep the unscaled column of L), and each
+// lane scales its own column once, at the end, with the 1 / sqrt(pivot) it
+// recorded at its own pivot.  The empty asm statements pin every register's
+// update to its pivot (otherwise the compiler sinks the E updates past later
+// pivots and keeps every multiplier alive).
+__device__ __forceinline__ void chol16_inv16(double* __restrict__ s, double* __restrict__ w, int r0) {
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15;
+  double a[16], e[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    a[i] = (i >= c) ? s[(r0 + i) * BTA_LDS_LD + r0 + c] : s[(r0 + c) * BTA_LDS_LD + r0 + i];
+    e[i] = (i == c) ? 1.0 : 0.0;
+  }
+  double myrs = 1.0, mysq = 1.0;
+#pragma unroll
+  for (int p = 0; p < 16; ++p) {
+    const double d = readlane_f64(a[p], p);  // pivot A_pp (lane p holds column p)
+    const double rs = rsq_f64(d);
+    if (c == p) {
+      myrs = rs;
+      mysq = d * rs;
+    }
+    const double mylp = (c > p) ? a[p] * rs : 0.0;  // L_cp = A_pc / sqrt(d) (symmetry)
+    e[p] *= rs;                                     // row p of E is final
+#pragma unroll
+    for (int i = p + 1; i < 16; ++i) {
+      const double bl = readlane_f64(a[i], p) * rs;  // L_ip
+      a[i] = fma(-bl, mylp, a[i]);
+      e[i] = fma(-bl, e[p], e[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      asm volatile("" : "+v"(a[i]));
+      asm volatile("" : "+v"(e[i]));
+    }
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      s[(r0 + i) * BTA_LDS_LD + r0 + c] = (i > c) ? a[i] * myrs : (i == c ? mysq : 0.0);
+      w[(r0 + i) * BTA_LDS_LD + r0 + c] = e[i];
+    }
+  }
+}
+
+__device__ __forceinline__ TileCholResult tile_chol_inv16(double* __restrict__ s, double* __restrict__ w, int mv) {
+  const int tid = threadIdx.x;
+  const int wv = tid >> 6, lane = tid & 63;
+  // identity outside the valid block, zero strict upper triangle, E = 0
+  for (int e = tid; e < 64 * 64; e += BTA_THREADS) {
+    const int i = e >> 6, j = e & 63;
+    if (i >= mv || j >= mv || j > i) s[i * BTA_LDS_LD + j] = (i == j) ? 1.0 : 0.0;
+    w[i * BTA_LDS_LD + j] = 0.0;
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int jb = 0; jb < 4; ++jb) {
+    const int r0 = 16 * jb;
+    // ---- phase 1
+    if (wv == 0) {
+      chol16_inv16(s, w, r0);
+    } else if (jb > 0) {
+      // T_jk = sum_{m=k}^{j-1} L_jm E_mk into w(j, k), k < j; jobs spread
+      // over waves 1..3 (at most 3 for jb == 3: k = 0 (3 terms), 1 (2), 2 (1))
+      for (int k = wv - 1; k < jb; k += 3) {
+        f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int m = k; m < jb; ++m) acc = mma16x16<false, 1>(s, r0, 16 * m, w, 16 * k, 16 * m, acc);
+        store16(w, r0, 16 * k, acc);
+      }
+    }
+    __syncthreads();
+    // ---- phase 2: waves 1..3, one 16x16 product each
+    if (wv > 0) {
+      const int q = wv - 1;  // q < 3 - jb: L_(jb+1+q), jb;  else E_jb, (q - (3 - jb))
+      f64x4 acc = f64x4{0.0, 0.0, 0.0, 0.0};
+      if (q < 3 - jb) {
+        const int ib = jb + 1 + q;
+        acc = mma16x16<false, 0>(s, 16 * ib, r0, w, r0, r0, acc);  // A_ib,jb E_jb,jb^T
+        store16(s, 16 * ib, r0, acc);
+      } else {
+        const int k = q - (3 - jb);
+        acc = mma16x16<true, 1>(w, r0, r0, w, 16 * k, r0, acc);    // -E_jj T_jk
+        store16(w, r0, 16 * k, acc);
+      }
+    }
+    __syncthreads();
+    // ---- phase 3: trailing updates A_ik -= L_ij L_kj^T, jb < k <= i
+    if (jb < 3) {
+      const int nr = 3 - jb, nt = nr * (nr + 1) / 2;
+      for (int t = wv; t < nt; t += 4) {
+        int rr = 0, cc = t;
+        while (cc > rr) {
+          cc -= rr + 1;
+          ++rr;
+        }
+        const int ib = jb + 1 + rr, kb = jb + 1 + cc;
+        f64x4 acc = load16(s, 16 * ib, 16 * kb);
+        acc = mma16x16<true, 0>(s, 16 * ib, r0, s, 16 * kb, r0, acc);
+        if (ib == kb) store16_lower(s, 16 * ib, 16 * kb, acc);
+        else store16(s, 16 * ib, 16 * kb, acc);
+      }
+      __syncthreads();
+    }
+  }
+  TileCholResult res;
+  res.fail = -1;
+  res.logsum = 0.0;
+  if (wv == 0) {
+    const double lii = s[lane * BTA_LDS_LD + lane];
+    const unsigned long long bad = __ballot(lane < mv && !(lii > 0.0));
+    res.fail = bad ? __ffsll((long long)bad) - 1 : -1;
+    double v = (lane < mv) ? 2.0 * log(lii) : 0.0;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    res.logsum = v;
+  }
+  __syncthreads();
+  return res;
+}
+
+#ifndef BTA_TILE_CHOL16
+#define BTA_TILE_CHOL16 1
+#endif
+
+// s: 64x64 LDS tile (stride BTA_LDS_LD) holding the SPD matrix in its lower
+// triangle; rows / cols >= mv are treated as identity.  On exit s holds L
+// (zero strict upper) and w holds L^-1 (lower).  scratch: >= 80 doubles of
+// LDS (used by the 8-wide panel variant only).
+__device__ __forceinline__ TileCholResult tile_chol_inv(double* __restrict__ s, double* __restrict__ w,
+                                                         double* __restrict__ scratch, int mv) {
+#if BTA_TILE_CHOL16
+  (void)scratch;
+  return tile_chol_inv16(s, w, mv);
+#else
+  return tile_chol_inv_p8(s, w, scratch, mv);
+#endif
 }

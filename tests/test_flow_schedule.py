@@ -35,13 +35,13 @@ def next_end(k, nT):
     return min(k + SPAN, tail) if k < tail else k + 1
 
 
-def schedule(nT, nTL, nTa, helpers, delta):
+def schedule(nT, nTL, nTa, helpers, delta, near_div=0):
     lib = _lib.load()
     n = C.c_int64()
-    assert lib.bta_flow_schedule(nT, nTL, nTa, helpers, delta, None, 0, None, C.byref(n)) == 0
+    assert lib.bta_flow_schedule(nT, nTL, nTa, helpers, delta, near_div, None, 0, None, C.byref(n)) == 0
     tasks = (C.c_uint32 * max(1, n.value))()
     off = (C.c_int * (helpers + 1))()
-    assert lib.bta_flow_schedule(nT, nTL, nTa, helpers, delta, tasks, n.value, off, None) == 0
+    assert lib.bta_flow_schedule(nT, nTL, nTa, helpers, delta, near_div, tasks, n.value, off, None) == 0
     words = list(tasks)[: n.value]
     lists = []
     for h in range(helpers):
@@ -316,10 +316,11 @@ def test_simulator_catches_planted_schedule_mistakes():
         Sim(D, L, B).run(bad, 0)
 
 
-@pytest.mark.parametrize("nT,nTL,nTa,helpers", [(32, 32, 1, 255), (32, 0, 0, 127), (16, 16, 1, 40),
-                                                 (32, 0, 1, 255), (71, 0, 0, 255), (71, 0, 1, 255),
-                                                 (71, 71, 1, 255), (71, 0, 0, 200), (40, 0, 0, 511)])
-def test_list_order_at_production_shape(nT, nTL, nTa, helpers):
+@pytest.mark.parametrize("nT,nTL,nTa,helpers,near", [(32, 32, 1, 255, 0), (32, 0, 0, 127, 0), (16, 16, 1, 40, 0),
+                                                      (32, 0, 1, 255, 0), (71, 0, 0, 255, 0), (71, 0, 1, 255, 0),
+                                                      (71, 71, 1, 255, 0), (71, 0, 0, 200, 0), (40, 0, 0, 511, 0),
+                                                      (32, 32, 1, 255, 4), (32, 0, 0, 255, 2), (32, 32, 0, 255, 3)])
+def test_list_order_at_production_shape(nT, nTL, nTa, helpers, near):
     """The list order (delta 0) for panels of b = 2048 and b = 4500 (71 tile
     columns) with full-device helper counts, blocks with and without L: no
     interleaving deadlocks and the factor is right (small tiles keep the
@@ -329,7 +330,7 @@ def test_list_order_at_production_shape(nT, nTL, nTa, helpers):
     ts = 4 if nT <= 32 else 2
     b = nT * ts
     D, L, B = _panel(b, min(3, ts) if nTa else 0, nTL > 0, 5)
-    lists = schedule(nT, nTL, nTa, helpers, 0)
+    lists = schedule(nT, nTL, nTa, helpers, 0, near)
     s = Sim(D, L, B, ts=ts)
     s.run(lists, 3)
     Lf = np.linalg.cholesky(D)
