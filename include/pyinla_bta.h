@@ -53,6 +53,7 @@ extern "C" {
 #define BTA_E_TIMEOUT (-3)  /* a bounded in-kernel wait timed out */
 #define BTA_E_LIB (-4)      /* a rocBLAS call failed */
 #define BTA_E_NORES (-5)    /* the workspace lacks the reservation the call needs */
+#define BTA_E_SCRATCH (-6)  /* the shape exceeds the fixed scratch of bta_potrf_noalloc */
 
 typedef struct bta_ws* bta_ws_t;
 
@@ -97,6 +98,23 @@ int bta_ws_prepare(bta_ws_t ws, double* D, double* L, double* B, double* T, int 
  * bta_ws_status (info), as LAPACK's potrf info. */
 int bta_potrf(bta_ws_t ws, double* D, double* L, double* B, double* T, int n, int b, int a,
               void* stream);
+
+/* bta_potrf without a workspace and without any device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under hipGraph stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same factorization and outputs as bta_potrf, computed by
+ * the per-tile-column panel path; every buffer beyond the caller's blocks and
+ * outputs lives in a fixed device array of the library (3.1 MiB per GPU, held
+ * until the process ends).  Limits: b <= 4096, a <= 64, n * ceil(b/64) + 1 <=
+ * 131072 and n < 2^20; larger shapes return BTA_E_SCRATCH before anything is
+ * enqueued.  Runs on the current HIP device (pointers must live there).
+ * Outputs, device pointers, each may be NULL: logdet[0] as bta_ws_status
+ * reports it, info[0] (0, or 1 + global row of the first non-positive pivot),
+ * err[0] (BTA_E_TIMEOUT if a bounded wait expired, else 0).  All calls on one
+ * GPU share the array: calls that overlap on one GPU (different streams, or
+ * threads without ordering) race and must not be issued. */
+int bta_potrf_noalloc(double* D, double* L, double* B, double* T, int n, int b, int a, double* logdet,
+                      int64_t* info, int* err, void* stream);
 
 /* Batched factorization of independent problems that may differ in (n, b, a)
  * (the 2 dim(theta) + 1 finite-difference evaluations, prior / conditional

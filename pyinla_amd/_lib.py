@@ -25,6 +25,7 @@ SIGNATURES = {
     "bta_ws_prepare": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _I]),
     "bta_logdet": (_I, [_P, _P, _P, _I, _I, _I, _P, _P]),
     "bta_potrf": (_I, [_P, _P, _P, _P, _P, _I, _I, _I, _P]),
+    "bta_potrf_noalloc": (_I, [_P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P]),
     "bta_ws_status": (_I, [_P, C.POINTER(C.c_int64), C.POINTER(_D), C.POINTER(_I), _P]),
     "bta_ws_logdet_ptr": (_P, [_P]),
     "bta_selftest_mfma": (_I, [_P, _P]),
@@ -49,7 +50,8 @@ SIGNATURES = {
 }
 
 BTA_E = {-1: "invalid argument", -2: "shape does not match the workspace", -3: "in-kernel wait timed out",
-         -4: "rocBLAS call failed", -5: "the workspace lacks the reservation this call needs"}
+         -4: "rocBLAS call failed", -5: "the workspace lacks the reservation this call needs",
+         -6: "the shape exceeds the fixed device scratch of bta_potrf_noalloc"}
 
 # workspace flags (include/pyinla_bta.h)
 WS_GRAPH, WS_SOLVE, WS_SELINV = 1, 2, 4

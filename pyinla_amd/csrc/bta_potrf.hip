@@ -17,8 +17,10 @@
 // and the arrowhead tip is factored at the end.  Per-tile log-pivot sums are
 // reduced in a fixed order, so the log-determinant is deterministic.
 #include <algorithm>
+#include <mutex>
 
 #include "bta_internal.h"
+#include "../../include/pyinla_bta.h"
 #include "bta_tile_chol.h"
 
 namespace bta {
@@ -47,6 +49,7 @@ struct PanelArgs {
   int* err;
   long long row0;            // global row of the block's first row
   unsigned long long spin_limit;
+  unsigned long long blk;    // mixed into the flag value: flags shared by several blocks stay distinct
   int b, a, nT, nTL, nTa, k;
 };
 
@@ -101,7 +104,7 @@ __host__ __device__ inline int panel_grid(int nT, int nTL, int nTa, int k) {
 __device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* __restrict__ sA,
                                            double* __restrict__ sB, double* __restrict__ sScr) {
   const TileRef t = decode_panel_tile(p, idx);
-  const unsigned long long ep = *p.epoch;
+  const unsigned long long ep = (*p.epoch << 20) | p.blk;
   const int b = p.b, k = p.k;
 
   if (t.kind == 3) {  // zero a strict-upper tile of D
@@ -208,6 +211,7 @@ __device__ __forceinline__ PanelArgs panel_args_of(const ProbDesc& d, int i, int
   p.err = d.err;
   p.row0 = (long long)i * b;
   p.spin_limit = d.spin_limit;
+  p.blk = 0;
   p.b = b;
   p.a = a;
   p.nT = d.nT;
@@ -537,6 +541,7 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     p.err = ws.err;
     p.row0 = (long long)i * b;
     p.spin_limit = ws.spin_limit;
+    p.blk = 0;
     p.b = b;
     p.a = a;
     p.nT = nT;
@@ -757,3 +762,127 @@ int BatchPlan::enqueue(hipStream_t st) const {
 }
 
 }  // namespace bta
+
+// ------------------------------------------------ factorization without allocation
+//
+// bta_potrf_noalloc runs the per-column panel path above (k_panel_step and the
+// 64x64-tile block updates, whose job lists travel as kernel arguments) with
+// every buffer beyond the caller's blocks and outputs in one file-scope device
+// array, `scratch`, of fixed size.  Layout, in doubles:
+//   [0] call counter (epoch)  [1] first failing pivot  [2] error bits  [3] logdet
+//   [8, 8 + NA_MAX_NT)                    readiness flags of the tile columns,
+//                                         shared by all blocks of a call (the
+//                                         flag value carries the block index)
+//   NA_TINV  (4096)                        inverse of the 64x64 tip tile
+//   NA_WINV  (NA_MAX_NT * 4096)            inverses of the current block's
+//                                         diagonal tiles (rewritten per block)
+//   NA_PART  (NA_MAX_PARTS)                per-tile log-pivot sums of the call
+// The counter starts at zero (static storage) and only grows, so flags never
+// need clearing; the pivot and error words are reset by k_begin_call.
+namespace bta {
+
+static constexpr int NA_MAX_NT = 64;          // b <= 4096
+static constexpr int NA_MAX_PARTS = 1 << 17;  // n * ceil(b / 64) + 1 tile log-sums
+static constexpr size_t NA_TINV = 8 + NA_MAX_NT;
+static constexpr size_t NA_WINV = NA_TINV + 4096;
+static constexpr size_t NA_PART = NA_WINV + (size_t)NA_MAX_NT * 4096;
+static constexpr size_t NA_TOTAL = NA_PART + NA_MAX_PARTS;  // 3.1 MiB
+
+static __device__ __align__(256) double scratch[NA_TOTAL];
+
+// Device address of `scratch` per device, taken once with that device current.
+static std::mutex na_mu;
+static double* na_ptr[256];
+
+static int noalloc_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= 256) return BTA_E_ARG;
+  std::lock_guard<std::mutex> lk(na_mu);
+  if (!na_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    if (!ptr) return (int)hipErrorInvalidSymbol;
+    na_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = na_ptr[dev];
+  return 0;
+}
+
+// Caller-visible results of the call: logdet, LAPACK-style info, error code.
+__global__ void k_noalloc_results(const double* sc, double* logdet, int64_t* info, int* err) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long inf = reinterpret_cast<const unsigned long long*>(sc)[1];
+  const int e = reinterpret_cast<const int*>(sc + 2)[0];
+  if (logdet) logdet[0] = sc[3];
+  if (info) info[0] = (inf == kInfoNone) ? 0 : (int64_t)inf + 1;
+  if (err) err[0] = e ? BTA_E_TIMEOUT : 0;
+}
+
+}  // namespace bta
+
+extern "C" int bta_potrf_noalloc(double* D, double* L, double* B, double* T, int n, int b, int a, double* logdet,
+                                 int64_t* info, int* err, void* stream) {
+  using namespace bta;
+  if (!D || n < 1 || b < 1 || a < 0 || a > 64) return BTA_E_ARG;
+  if (n > 1 && !L) return BTA_E_ARG;
+  if (a > 0 && (!B || !T)) return BTA_E_ARG;
+  const int nT = (b + 63) / 64, nTa = (a + 63) / 64;
+  if (nT > NA_MAX_NT || (long long)n * nT + 1 > NA_MAX_PARTS || n >= (1 << 20)) return BTA_E_SCRATCH;
+  double* sc = nullptr;
+  int rc = noalloc_scratch(&sc);
+  if (rc) return rc;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto* u = reinterpret_cast<unsigned long long*>(sc);
+  unsigned long long* epoch = u;
+  unsigned long long* inf = u + 1;
+  int* er = reinterpret_cast<int*>(u + 2);
+  hipLaunchKernelGGL(k_begin_call, dim3(1), dim3(64), 0, st, epoch, inf, er);
+  for (int i = 0; i < n; ++i) {
+    PanelArgs p;
+    p.D = D + (int64_t)i * b * b;
+    p.L = (i < n - 1) ? L + (int64_t)i * b * b : nullptr;
+    p.B = (a > 0) ? B + (int64_t)i * a * b : nullptr;
+    p.Winv = sc + NA_WINV;
+    p.logpart = sc + NA_PART + (int64_t)i * nT;
+    p.flags = u + 8;
+    p.epoch = epoch;
+    p.info = inf;
+    p.err = er;
+    p.row0 = (long long)i * b;
+    p.spin_limit = 1ull << 24;
+    p.blk = (unsigned long long)i;
+    p.b = b;
+    p.a = a;
+    p.nT = nT;
+    p.nTL = p.L ? nT : 0;
+    p.nTa = p.B ? nTa : 0;
+    for (int k = 0; k < nT; ++k) {
+      p.k = k;
+      hipLaunchKernelGGL(k_panel_step, dim3(panel_grid(nT, p.nTL, p.nTa, k)), dim3(BTA_THREADS), 0, st, p);
+    }
+    UpdJobs jobs;
+    jobs.njobs = 0;
+    if (i < n - 1) {
+      double* Dn = D + (int64_t)(i + 1) * b * b;
+      jobs.job[jobs.njobs++] = UpdJob{Dn, p.L, p.L, b, b, b, b, b, b, 1, 0, 0};
+      if (a > 0) {
+        double* Bn = B + (int64_t)(i + 1) * a * b;
+        jobs.job[jobs.njobs++] = UpdJob{Bn, p.B, p.L, a, b, b, b, b, b, 0, 0, 0};
+      }
+    }
+    if (a > 0) jobs.job[jobs.njobs++] = UpdJob{T, p.B, p.B, a, a, b, a, b, b, 1, 0, 0};
+    launch_gemm_jobs(jobs, st);
+  }
+  int npart = n * nT;
+  if (a > 0) {
+    hipLaunchKernelGGL(k_tip_potrf, dim3(1), dim3(BTA_THREADS), 0, st, T, a, sc + NA_PART + npart, inf,
+                       (long long)n * b, sc + NA_TINV);
+    npart += 1;
+  }
+  hipLaunchKernelGGL(k_logdet_reduce, dim3(1), dim3(BTA_THREADS), 0, st, sc + NA_PART, npart, sc + 3);
+  hipLaunchKernelGGL(k_noalloc_results, dim3(1), dim3(64), 0, st, sc, logdet, info, err);
+  return (int)hipGetLastError();
+}
