@@ -120,7 +120,20 @@ def cpu_baseline(args, prob_full):
     t_f = float(np.median(times))
     d = 4
     t_grad_full = (2 * d + 1) * t_f * (args.nt / nts)
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except Exception:
+        nproc = os.cpu_count() or 1
+    cpu_model = ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": 1.0 / t_grad_full, "unit": "evals/s", "cores": int(threads), "kind": "port",
+            "host_cpus_nproc": int(nproc), "host_cpu_model": cpu_model,
             "sample": (f"oracle f(theta) (numpy/scipy BTA restatement, LAPACK/BLAS with {threads} threads) "
                        f"at nt={nts} on the same {args.nx}x{args.ny} mesh, median of 3 = {t_f:.3f} s; "
                        f"FD gradient = {2 * d + 1} f evaluations (reference order), extrapolated linearly "
@@ -313,6 +326,7 @@ def main():
 
     # plain GPU FD path (9 f evaluations, reference order), step 1e-3
     t_fd, grad_rel = float("nan"), float("nan")
+    grad_rel4, t_fdb, fd_wave = float("nan"), float("nan"), 0
     if not args.skip_fd:
         f_fd, g_fd = model.fd_gradient(theta, eps=1e-3)  # warm
         torch.cuda.synchronize()
@@ -323,6 +337,22 @@ def main():
         t_fd = (time.perf_counter() - t0) / args.fd_steps
         log(f"FD f+grad: {t_fd * 1e3:.1f} ms/step")
         grad_rel = float(np.abs(g_adj - g_fd).max() / max(np.abs(g_fd).max(), 1e-300))
+        # the same 2 d + 1 = 9-point central FD with step 1e-4 (the gradient check)
+        f_fd4, g_fd4 = model.fd_gradient(theta, eps=1e-4)
+        grad_rel4 = float(np.abs(g_adj - g_fd4).max() / max(np.abs(g_fd4).max(), 1e-300))
+        log(f"adjoint vs 9-point FD (step 1e-4): rel {grad_rel4:.2e}")
+        # batched FD: the 18 factorizations in batched, graph-replayed waves
+        wave = args.fd_wave if args.fd_wave > 0 else None
+        model.fd_gradient_batched(theta, eps=1e-3, wave=wave)  # pool + graph set-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f_fdb, g_fdb = model.fd_gradient_batched(theta, eps=1e-3, wave=wave)
+        torch.cuda.synchronize()
+        t_fdb = time.perf_counter() - t0
+        fd_wave = len(model._fd_pool[0])
+        model._fd_pool = None  # release the pool before the remaining measurements
+        torch.cuda.empty_cache()
+        log(f"batched FD f+grad: {t_fdb * 1e3:.1f} ms/step (waves of {fd_wave} stencil points)")
 
     # live per-kernel-class timing (HIP events on the launch stream) of one
     # adjoint evaluation
@@ -407,7 +437,10 @@ def main():
                                f"({args.nx}x{args.ny} grid), nb={a}, dim_theta=4",
                    "nt": n, "ns": b, "nb": a, "dim_theta": 4, "observations": int(prob.A.shape[0])},
         "fd": {"value": _num(1.0 / t_fd), "unit": "evals/s", "ms_per_step": _num(t_fd * 1e3),
-               "adjoint_speedup": _num(t_fd / t_adj), "grad_rel_diff_adjoint_vs_fd1e-3": _num(grad_rel)},
+               "adjoint_speedup": _num(t_fd / t_adj), "grad_rel_diff_adjoint_vs_fd1e-3": _num(grad_rel),
+               "grad_rel_diff_adjoint_vs_fd9pt_step1e-4": _num(grad_rel4),
+               "batched_ms_per_step": _num(t_fdb * 1e3), "batched_wave": fd_wave,
+               "adjoint_speedup_vs_batched": _num(t_fdb / t_adj)},
         "pobtaf_tflops": potrf_tflops,
         "pobtaf_frac_peak": potrf_tflops / PEAK_FP64_TFLOPS if potrf_tflops else None,
         "pobtasi_tflops": potri_tflops,

@@ -184,9 +184,10 @@ int bta_logdet(bta_ws_t ws, const double* D, const double* T, int n, int b, int 
  * updates and solves the chain waits for, go to a pool of helpers / value
  * helpers that own nothing else (default 8; 1..64). */
 #define BTA_OPT_FLOW_NEAR 8
-/* 64x64 grouped GEMM main loop: 0 (default) = two LDS buffers with loads
- * issued two K chunks ahead, 1 = one LDS buffer (the earlier form, kept for
- * comparison). */
+/* 64x64 grouped GEMM main loop: 0 (default) = one LDS buffer, the next K
+ * chunk loaded into registers while the current one is multiplied; 1 = two
+ * LDS buffers with loads issued two K chunks ahead (slower on MI355X: 470 us
+ * against 395 us for NN 2000^3; kept for comparison). */
 #define BTA_OPT_GEMM_KERNEL 9
 int bta_ws_set_option(bta_ws_t ws, int option, int value);
 
@@ -313,8 +314,8 @@ int bta_kron_contract(const double* D, const double* L, const double* x, int n, 
 /* Diagnostic (no reference counterpart): C = beta C + alpha op(A) op(B) on row-
  * major FP64 matrices (op(X) = X, or X^T when ta / tb = 1; C m x n, K = k)
  * through the grouped-GEMM kernel the selected inversion uses: variant 0 =
- * the pipelined 64x64 kernel, 1 = its single-buffered predecessor, 2 = 128x128
- * tiles.  Builds a one-job plan (allocates), launches on `stream` and
+ * the default 64x64 kernel, 1 = the 64x64 kernel with two LDS buffers, 2 =
+ * 128x128 tiles.  Builds a one-job plan (allocates), launches on `stream` and
  * synchronises it; for tests and kernel measurements. */
 int bta_gemm(const double* A, int lda, int ta, const double* B, int ldb, int tb, double* C, int ldc, int m, int n,
              int k, double alpha, double beta, int variant, void* stream);

@@ -127,7 +127,9 @@ struct GemvJob {
 // once (hipMalloc + hipMemcpy, never during stream capture), then launched any
 // number of times, including from inside a captured hipGraph.
 struct DevPlan {
-  struct Entry { int job_off, pre_off, njobs, tiles, ts; };
+  // vec: every operand has an even leading dimension and a 16-byte aligned
+  // base (the pipelined kernel's paired loads need both)
+  struct Entry { int job_off, pre_off, njobs, tiles, ts, vec; };
   GemmJob* d_jobs = nullptr;
   int* d_prefix = nullptr;
   size_t bytes = 0;  // device bytes of the job table and prefix sums

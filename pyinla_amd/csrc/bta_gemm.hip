@@ -122,7 +122,7 @@ struct ChunkPrefetch {
   }
 };
 
-__global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped_v1(const GemmJob* __restrict__ jobs,
+__global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(const GemmJob* __restrict__ jobs,
                                                              const int* __restrict__ prefix, int njobs) {
   __shared__ double sA[64 * BTA_GEMM_LD];
   __shared__ double sB[64 * BTA_GEMM_LD];
@@ -233,17 +233,27 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped_v1(
 
 // ------------------------------------------- 64x64 tiles, pipelined K loop
 //
-// The production form of the grouped GEMM.  K advances in chunks of 16; the
+// Variant of the grouped GEMM (BTA_OPT_GEMM_KERNEL 1; measured slower on
+// MI355X than the single-buffered k_gemm_grouped above, 470 us against 395 us
+// for NN 2000^3, and kept for comparison).  K advances in chunks of 16; the
 // A chunk (64 rows of op(A) x 16) is kept in LDS as sA[i][k] (odd stride 17)
 // and the B chunk (16 x 64 columns of op(B)) as sB[k][j] (stride 68), so the
 // MFMA operand reads of a wave (16 rows x 4 k) hit distinct banks.  Two LDS
 // buffers and two register sets: while chunk t is multiplied out of one
 // buffer, chunk t+1 (loaded two chunks earlier) is stored to the other and
 // the loads of chunk t+3 are issued -- each global load has two chunks of
-// MFMA work to arrive.  Alpha is applied once, when the A chunk is stored to
-// LDS.  Interior chunks with even leading dimensions and 16-byte aligned
-// addresses load pairs of doubles; boundary chunks (ragged edges, partial K,
-// triangular masks) load element by element with zero fill.
+// MFMA work to arrive.
+//
+// Every chunk is loaded the same way, as pairs of doubles from addresses
+// clamped into the operand (rows past the edge of the matrix, K past the end
+// of the segment's range), and everything else -- zero fill at ragged edges
+// and partial chunks, triangular masks, alpha -- is applied when the chunk is
+// stored to LDS.  The load sequence is then the same straight-line code for
+// every chunk, so the compiler can count the loads in flight and wait for one
+// register set only.  Past the last chunk the loop re-loads the first one
+// into the idle set (never stored).  Needs even leading dimensions and
+// 16-byte aligned operands (DevPlan::launch falls back to the single-buffered
+// kernel otherwise): a clamped pair then never leaves its matrix row.
 
 #define G2_KC 16
 #define G2_LDA (G2_KC + 1)
@@ -253,102 +263,81 @@ typedef double f64x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(1))) f64x2 gf64x2;
 
 // Four elements of one 64 x 16 operand chunk per thread.  form 0 ("K
-// contiguous"): element (r, k) at g[r * ld + k], the thread owns rows
-// t/8 and t/8 + 32, columns 2 (t%8) and 2 (t%8) + 1; form 1 ("rows
-// contiguous"): element (r, k) at g[k * ld + r], the thread owns k = t/32
-// and t/32 + 8, rows 2 (t%32) and 2 (t%32) + 1.
+// contiguous"): element (r, k) at g[r * ld + k], the thread owns rows t/8 and
+// t/8 + 32, columns 2 (t%8) and 2 (t%8) + 1; form 1 ("rows contiguous"):
+// element (r, k) at g[k * ld + r], the thread owns k = t/32 and t/32 + 8, rows
+// 2 (t%32) and 2 (t%32) + 1.  rows / kw: valid rows / K columns of the chunk.
 struct G2Op {
   double r[4];
-  __device__ __forceinline__ void load(const double* __restrict__ g0, int64_t ld, int form, int rows, int kw, int mask,
-                                       int i0, int k0) {
+  // (row, k) inside the chunk of element u of pair q, and the memory offset
+  // (in elements, from the chunk origin) of pair q before clamping
+  __device__ __forceinline__ static void coord(int form, int q, int u, int& i, int& k) {
+    const int t = threadIdx.x;
+    const int i0 = (t >> 3) + 32 * q, k0 = 2 * (t & 7) + u;  // form 0
+    const int i1 = 2 * (t & 31) + u, k1 = (t >> 5) + 8 * q;  // form 1
+    i = form ? i1 : i0;
+    k = form ? k1 : k0;
+  }
+  // The same two paired loads for either form: only the addresses differ.
+  __device__ __forceinline__ void load(const double* __restrict__ g0, int64_t ld, int form, int rows, int kw) {
     const int t = threadIdx.x;
     const gdouble* g = gview(g0);
-    mask = chunk_mask(mask, i0, rows, k0, kw);
-    const bool fast = mask == MASK_NONE && rows >= 64 && kw >= G2_KC && (ld & 1) == 0 &&
-                      (reinterpret_cast<uintptr_t>(g0) & 15) == 0;
-    if (form == 0) {
-      const int r0 = t >> 3, kk = 2 * (t & 7);
-      if (fast) {
+    const int kc = min(2 * (t & 7), (kw - 1) & ~1);
+    const int rc = min(2 * (t & 31), (rows - 1) & ~1);
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const f64x2 v = *(const gf64x2*)(g + (int64_t)(r0 + 32 * q) * ld + kk);
-          r[2 * q] = v.x;
-          r[2 * q + 1] = v.y;
-        }
-        return;
-      }
-#pragma unroll
-      for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int rr = r0 + 32 * q, k = kk + u;
-          r[2 * q + u] = (rr < rows && k < kw && !masked_out(mask, i0 + rr, k0 + k)) ? g[(int64_t)rr * ld + k] : 0.0;
-        }
-    } else {
-      const int k0t = t >> 5, rr = 2 * (t & 31);
-      if (fast) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const f64x2 v = *(const gf64x2*)(g + (int64_t)(k0t + 8 * q) * ld + rr);
-          r[2 * q] = v.x;
-          r[2 * q + 1] = v.y;
-        }
-        return;
-      }
-#pragma unroll
-      for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int k = k0t + 8 * q, ri = rr + u;
-          r[2 * q + u] = (ri < rows && k < kw && !masked_out(mask, i0 + ri, k0 + k)) ? g[(int64_t)k * ld + ri] : 0.0;
-        }
+    for (int q = 0; q < 2; ++q) {
+      const int rr = min((t >> 3) + 32 * q, rows - 1);
+      const int kq = min((t >> 5) + 8 * q, kw - 1);
+      const int64_t off = form ? (int64_t)kq * ld + rc : (int64_t)rr * ld + kc;
+      const f64x2 v = *(const gf64x2*)(g + off);
+      r[2 * q] = v.x;
+      r[2 * q + 1] = v.y;
     }
+  }
+  // value of element (q, u) after the zero fill and the triangular mask,
+  // evaluated without branches (bitwise, not short-circuit)
+  __device__ __forceinline__ double val(int form, int q, int u, int rows, int kw, int mask, int i0, int k0) const {
+    int i, k;
+    coord(form, q, u, i, k);
+    const int ig = i0 + i, kg = k0 + k;
+    const bool out = (mask == MASK_K_GT_I) & (kg > ig) | (mask == MASK_K_LT_I) & (kg < ig);
+    const bool keep = (i < rows) & (k < kw) & !out;
+    return keep ? r[2 * q + u] : 0.0;
   }
   // A operand: sA[i][k] scaled by alpha
-  __device__ __forceinline__ void store_a(double* __restrict__ sA, int form, double alpha) const {
-    const int t = threadIdx.x;
-    if (form == 0) {
-      const int r0 = t >> 3, kk = 2 * (t & 7);
+  __device__ __forceinline__ void store_a(double* __restrict__ sA, int form, double alpha, int rows, int kw, int mask,
+                                          int i0, int k0) const {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        sA[(r0 + 32 * q) * G2_LDA + kk] = alpha * r[2 * q];
-        sA[(r0 + 32 * q) * G2_LDA + kk + 1] = alpha * r[2 * q + 1];
-      }
-    } else {
-      const int k0t = t >> 5, rr = 2 * (t & 31);
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        sA[rr * G2_LDA + k0t + 8 * q] = alpha * r[2 * q];
-        sA[(rr + 1) * G2_LDA + k0t + 8 * q] = alpha * r[2 * q + 1];
+      for (int u = 0; u < 2; ++u) {
+        int i, k;
+        coord(form, q, u, i, k);
+        sA[i * G2_LDA + k] = alpha * val(form, q, u, rows, kw, mask, i0, k0);
       }
-    }
   }
   // B operand: sB[k][j]
-  __device__ __forceinline__ void store_b(double* __restrict__ sB, int form) const {
-    const int t = threadIdx.x;
-    if (form == 0) {
-      const int r0 = t >> 3, kk = 2 * (t & 7);
+  __device__ __forceinline__ void store_b(double* __restrict__ sB, int form, int cols, int kw, int mask, int j0,
+                                          int k0) const {
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        sB[kk * G2_LDB + r0 + 32 * q] = r[2 * q];
-        sB[(kk + 1) * G2_LDB + r0 + 32 * q] = r[2 * q + 1];
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        int i, k;
+        coord(form, q, u, i, k);
+        sB[k * G2_LDB + i] = val(form, q, u, cols, kw, mask, j0, k0);
       }
-    } else {
-      const int k0t = t >> 5, rr = 2 * (t & 31);
-#pragma unroll
-      for (int q = 0; q < 2; ++q) *(f64x2*)&sB[(k0t + 8 * q) * G2_LDB + rr] = f64x2{r[2 * q], r[2 * q + 1]};
-    }
   }
 };
 
-// One staged chunk: both operands and the segment it belongs to.
+// One staged chunk: both operands and the chunk they hold.
 struct G2Stage {
   G2Op a, b;
-  int seg;
+  int seg, k0, kw;
 };
 
-__global__ void __launch_bounds__(BTA_THREADS, 4) k_gemm_grouped(const GemmJob* __restrict__ jobs,
-                                                                const int* __restrict__ prefix, int njobs) {
+__global__ void __launch_bounds__(BTA_THREADS, 4) k_gemm_grouped_db(const GemmJob* __restrict__ jobs,
+                                                                   const int* __restrict__ prefix, int njobs) {
   __shared__ double sA[2][64 * G2_LDA];
   __shared__ double sB[2][G2_KC * G2_LDB];
   const int jid = find_job(prefix, njobs, blockIdx.x);
@@ -405,18 +394,20 @@ __global__ void __launch_bounds__(BTA_THREADS, 4) k_gemm_grouped(const GemmJob* 
   auto fetch = [&](const Cur& c, G2Stage& st) {
     const GemmSeg& sg = g.seg[c.seg];
     const int kw = min(G2_KC, c.ke - c.k0);
-    const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
-    const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
-    if (sg.ta == 0) st.a.load(sg.A + (int64_t)(tr * 64) * sg.lda + c.k0, sg.lda, 0, rows, kw, ma, tr * 64, c.k0);
-    else st.a.load(sg.A + (int64_t)c.k0 * sg.lda + tr * 64, sg.lda, 1, rows, kw, ma, tr * 64, c.k0);
-    if (sg.tb == 1) st.b.load(sg.B + (int64_t)(tc * 64) * sg.ldb + c.k0, sg.ldb, 0, cols, kw, mb, tc * 64, c.k0);
-    else st.b.load(sg.B + (int64_t)c.k0 * sg.ldb + tc * 64, sg.ldb, 1, cols, kw, mb, tc * 64, c.k0);
+    const int64_t oa = sg.ta ? (int64_t)c.k0 * sg.lda + tr * 64 : (int64_t)(tr * 64) * sg.lda + c.k0;
+    const int64_t ob = sg.tb ? (int64_t)(tc * 64) * sg.ldb + c.k0 : (int64_t)c.k0 * sg.ldb + tc * 64;
+    st.a.load(sg.A + oa, sg.lda, sg.ta, rows, kw);
+    st.b.load(sg.B + ob, sg.ldb, 1 - sg.tb, cols, kw);
     st.seg = c.seg;
+    st.k0 = c.k0;
+    st.kw = kw;
   };
   auto stage = [&](const G2Stage& st, int buf) {
     const GemmSeg& sg = g.seg[st.seg];
-    st.a.store_a(sA[buf], sg.ta, sg.alpha);
-    st.b.store_b(sB[buf], sg.tb == 1 ? 0 : 1);
+    const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
+    const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
+    st.a.store_a(sA[buf], sg.ta, sg.alpha, rows, st.kw, ma, tr * 64, st.k0);
+    st.b.store_b(sB[buf], sg.tb == 1 ? 0 : 1, cols, st.kw, mb, tc * 64, st.k0);
   };
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
@@ -435,54 +426,43 @@ __global__ void __launch_bounds__(BTA_THREADS, 4) k_gemm_grouped(const GemmJob* 
       acc.c[1][1] = mfma_f64(a1, b1, acc.c[1][1]);
     }
   };
-  Cur c0 = first();
-  if (c0.seg < g.nseg) {
+  // number of chunks of the tile (all segments)
+  int nch = 0;
+  for (int sid = 0; sid < g.nseg; ++sid) {
+    int kb, ke;
+    seg_range(sid, kb, ke);
+    if (ke > kb) nch += (ke - kb + G2_KC - 1) / G2_KC;
+  }
+  if (nch > 0) {
+    // Chunk t sits in LDS buffer t % 2 when it is multiplied; past the last
+    // chunk the cursor stays on the first one (loaded, never multiplied), so
+    // every pass issues the same loads and stores and only the MFMAs of a
+    // missing odd chunk are skipped.
+    const Cur c0 = first();
     G2Stage s0, s1;
-    fetch(c0, s0);
-    Cur cf = next(c0);
-    bool have1 = cf.seg < g.nseg, have2 = false;
-    if (have1) fetch(cf, s1);
+    Cur cf = c0;  // last chunk fetched (c0 again once past the end)
+    auto advance = [&]() {
+      const Cur c = next(cf);
+      cf = (c.seg < g.nseg) ? c : c0;
+    };
+    fetch(cf, s0);
+    advance();
+    fetch(cf, s1);
     stage(s0, 0);
-    if (have1) {
-      const Cur c2 = next(cf);
-      have2 = c2.seg < g.nseg;
-      if (have2) {
-        fetch(c2, s0);
-        cf = c2;
-      }
-    }
+    advance();
+    fetch(cf, s0);
     __syncthreads();
-    // invariant at the top: the current chunk is in buffer 0, the next in s1
-    // (have1), the one after in s0 (have2); cf is the last fetched chunk
-    for (;;) {
-      mma(0);
-      if (!have1) break;
-      stage(s1, 1);
-      bool have3 = false;
-      if (have2) {
-        const Cur c3 = next(cf);
-        have3 = c3.seg < g.nseg;
-        if (have3) {
-          fetch(c3, s1);
-          cf = c3;
-        }
-      }
+    for (int t = 0; t < nch; t += 2) {
+      mma(0);          // chunk t
+      stage(s1, 1);    // chunk t + 1
+      advance();
+      fetch(cf, s1);   // chunk t + 3
       lds_barrier();
-      mma(1);
-      if (!have2) break;
-      stage(s0, 0);
-      bool have4 = false;
-      if (have3) {
-        const Cur c4 = next(cf);
-        have4 = c4.seg < g.nseg;
-        if (have4) {
-          fetch(c4, s0);
-          cf = c4;
-        }
-      }
+      if (t + 1 < nch) mma(1);
+      stage(s0, 0);    // chunk t + 2
+      advance();
+      fetch(cf, s0);   // chunk t + 4
       lds_barrier();
-      have1 = have3;
-      have2 = have4;
     }
   }
   // epilogue
@@ -872,6 +852,14 @@ int DevPlan::build(const std::vector<GemmLaunch>& ls) {
     e.njobs = (int)l.jobs.size();
     e.tiles = l.tiles();
     e.ts = l.ts;
+    e.vec = 1;
+    for (const GemmJob& j : l.jobs)
+      for (int q = 0; q < j.nseg; ++q) {
+        const GemmSeg& sg = j.seg[q];
+        if ((sg.lda & 1) || (sg.ldb & 1) || (reinterpret_cast<uintptr_t>(sg.A) & 15) ||
+            (reinterpret_cast<uintptr_t>(sg.B) & 15))
+          e.vec = 0;
+      }
     launches.push_back(e);
     all.insert(all.end(), l.jobs.begin(), l.jobs.end());
     pre.insert(pre.end(), l.prefix.begin(), l.prefix.end());
@@ -894,8 +882,8 @@ void DevPlan::launch(int idx, hipStream_t st, int variant) const {
   if (e.ts == BIG_TS)
     hipLaunchKernelGGL(k_gemm_grouped_big, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
                        d_prefix + e.pre_off, e.njobs);
-  else if (variant == 1)
-    hipLaunchKernelGGL(k_gemm_grouped_v1, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
+  else if (variant == 1 && e.vec)
+    hipLaunchKernelGGL(k_gemm_grouped_db, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
                        d_prefix + e.pre_off, e.njobs);
   else
     hipLaunchKernelGGL(k_gemm_grouped, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,

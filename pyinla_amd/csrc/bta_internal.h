@@ -162,7 +162,7 @@ struct Workspace {
   int flow_helpers = 0;  // 0 = automatic
   int flow_delta = 0;    // 0: list-scheduled task order; >= 1: level order, far-tile update groups
   int flow_near = 8;     // near pool = helpers / flow_near
-  int gemm_variant = 0;  // 64x64 grouped GEMM: 0 = pipelined K loop, 1 = single-buffered
+  int gemm_variant = 0;  // 64x64 grouped GEMM: 0 = single LDS buffer, 1 = two LDS buffers
   int batch_lanes = 0;   // problems per wave of a batched factorization (0 = 6)
   // persistent-panel exchange buffers, sized for the workspace's arrowhead
   // (allocated by bta_ws_create when the flow panel can run)
