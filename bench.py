@@ -256,20 +256,30 @@ def bench_coregional(args, dev):
     th0 = prob.theta.copy()
     model.value_and_grad(th0)
     torch.cuda.synchronize()
-    r_adj = minimize(model, th0, gradient="adjoint", max_iter=args.lbfgs_iters)
+    log(f"coregional model resident: n={prob.nt} b={model.b} a={model.a}")
+    r_adj = minimize(model, th0, gradient="adjoint", max_iter=args.lbfgs_iters, verbose=True)
     torch.cuda.synchronize()
-    r_fd = minimize(model, th0, gradient="fd", max_iter=args.lbfgs_fd_iters)
+    r_fd = minimize(model, th0, gradient="fd", max_iter=args.lbfgs_fd_iters, verbose=True)
     torch.cuda.synchronize()
-    per_it_adj = r_adj["wall_time"] / max(r_adj["iterations"], 1)
-    per_it_fd = r_fd["wall_time"] / max(r_fd["iterations"], 1)
+    r_fdb = minimize(model, th0, gradient="fd_batched", max_iter=args.lbfgs_fd_iters, verbose=True,
+                     fd_wave=args.fd_wave if args.fd_wave > 0 else None)
+    torch.cuda.synchronize()
+    fd_wave = len(model._fd_pool[0]) if getattr(model, "_fd_pool", None) else 0
+
+    def summary(r):
+        it = max(r["iterations"], 1)
+        return {"iterations": r["iterations"], "objective_calls": r["objective_calls"], "wall_s": r["wall_time"],
+                "s_per_iter": r["wall_time"] / it, "s_per_objective_call": float(np.mean(r["objective_time"])),
+                "f": r["f"], "message": r["message"]}
+
+    adj, fd, fdb = summary(r_adj), summary(r_fd), summary(r_fdb)
+    fdb["wave"] = fd_wave
     return {"metric": "coregional L-BFGS wall time per iteration", "unit": "s",
             "config": {"workload": f"coregional nv=3 ns=1500 nt={args.nt} b=4500 nb=9 dim_theta=15"},
-            "adjoint": {"iterations": r_adj["iterations"], "objective_calls": r_adj["objective_calls"],
-                        "wall_s": r_adj["wall_time"], "s_per_iter": per_it_adj, "f": r_adj["f"],
-                        "message": r_adj["message"]},
-            "fd": {"iterations": r_fd["iterations"], "objective_calls": r_fd["objective_calls"],
-                   "wall_s": r_fd["wall_time"], "s_per_iter": per_it_fd, "f": r_fd["f"]},
-            "fd_over_adjoint_per_iter": per_it_fd / per_it_adj}
+            "adjoint": adj, "fd": fd, "fd_batched": fdb,
+            "fd_over_adjoint_per_objective_call": fd["s_per_objective_call"] / adj["s_per_objective_call"],
+            "fd_batched_over_adjoint_per_objective_call":
+                fdb["s_per_objective_call"] / adj["s_per_objective_call"]}
 
 
 def main():

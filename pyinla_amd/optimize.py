@@ -40,6 +40,9 @@ def minimize(model, theta0, gradient: str = "adjoint", max_iter: int = 100, maxc
             f, g = model.fd_gradient_batched(theta, eps=eps_gradient_f, wave=fd_wave)
         hist["objective_time"].append(time.perf_counter() - tic)
         hist["n_obj"] += 1
+        if verbose:
+            print(f"  objective call {hist['n_obj']:3d} ({gradient}): f={float(f):.8f} "
+                  f"in {hist['objective_time'][-1]:.2f} s", flush=True)
         last["grad"] = np.asarray(g, dtype=np.float64).copy()
         return float(f), np.asarray(g, dtype=np.float64)
 
