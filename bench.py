@@ -74,6 +74,8 @@ def parse():
                     help="first run a plain device-to-device copy of MIB MiB (a known-size kernel to scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters against)")
     ap.add_argument("--skip-fd", action="store_true", help="skip the FD comparison (profiling runs)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the solvers' launch options on Q_prior / Q_cond first (pyinla_amd.autotune)")
     ap.add_argument("--workload", default="adjoint",
                     choices=["adjoint", "fd-batch", "mixed-batch", "coregional-lbfgs"],
                     help="adjoint: the headline line; the others measure the batched FD stencil, the "
@@ -320,6 +322,12 @@ def main():
             s_.set_selinv_blas(bool(args.selinv_blas))
     theta = prob.theta.copy()
     log("model resident on GPU")
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tuned = model.autotune(theta)
+        tuned = {k: {n: v for n, v in c.items() if n != "times_ms"} for k, c in tuned.items()}
+        log(f"autotuned in {time.perf_counter() - t0:.1f}s: {tuned}")
 
     for w in range(args.warmup):
         f0, g_adj = model.value_and_grad(theta)
@@ -423,6 +431,30 @@ def main():
             pass
     roof["launches"] = cnt[dom]
     roof["avg_launch_ms"] = tot[dom] / max(cnt[dom], 1)
+    if roof["kernel"] == "k_panel_flow":
+        # the two matrices' panel launches run side by side on two streams, so
+        # the per-launch rate above is half the device's: also time the pair
+        # of factorizations by wall clock
+        pair = []
+        for _ in range(2):
+            model.assemble_prior(theta)
+            model.assemble_cond(theta)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cur = model._fork()
+            with torch.cuda.stream(model.s_prior):
+                model.prior.factorize("bt")
+            with torch.cuda.stream(model.s_cond):
+                model.cond.factorize("bta")
+            model._join(cur)
+            torch.cuda.synchronize()
+            pair.append(time.perf_counter() - t0)
+        t_pair = min(pair)
+        dev = (flops_potrf(n, b, a) + flops_potrf(n, b, 0)) / t_pair / 1e12
+        roof["concurrent_launches"] = 2
+        roof["pair_wall_ms"] = t_pair * 1e3
+        roof["device_achieved"] = dev
+        roof["device_frac"] = dev / PEAK_FP64_TFLOPS
     breakdown = {classes[c]: {"ms": tot[c], "launches": cnt[c]} for c in classes}
 
     cpu = None
@@ -458,6 +490,7 @@ def main():
         "kernel_ms": breakdown,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "autotuned": tuned,
         "f": f_adj,
     }
     print(json.dumps(out), flush=True)

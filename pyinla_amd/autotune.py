@@ -1,11 +1,23 @@
-"""Tile-shape autotuner for the FP64-MFMA GEMMs of the BTA solver.
+"""Autotuner of the BTA solver's launch options, timed on the solver's own matrix.
 
-For each GEMM class (block updates of the factorization; block inverses +
-selected inversion) the candidate output-tile edges (64, 128) are timed on the
-solver's own matrix with HIP events, the blocks being restored from a device
-copy before every trial, and the fastest is kept in the solver's workspace.
-Results are cached per (n, b, a) in memory and, if a path is given, in a JSON
-file.
+The search space is the workspace options of include/pyinla_bta.h that change
+speed but not results beyond rounding:
+
+  factorization       panel variant (persistent dataflow panel, or one launch
+                      per 64-wide tile column), the near-pool divisor of the
+                      persistent panel, the output tile of the block-update
+                      GEMMs of the per-column panel (64 / 128);
+  selected inversion  output tile of the grouped GEMM (64 / 128), its 64x64
+                      main loop (one or two LDS buffers), and whether the
+                      Sigma_{i+1,i} products go to rocBLAS dgemm or stay in the
+                      grouped kernel.
+
+Coordinate descent: one option at a time, the others at their best so far;
+each candidate is timed with HIP events (best of ``reps``, after one warm-up
+run that builds its plans) on the solver's blocks, restored from a device copy
+before every trial.  Results are cached per (n, b, a) in memory and, if a path
+is given, in a JSON file, and applied to the workspace.  No reference
+counterpart: the reference runs Serinv with fixed settings.
 """
 from __future__ import annotations
 
@@ -18,6 +30,17 @@ from . import _lib
 
 OPT_TILE_UPDATE = 1
 OPT_TILE_GROUPED = 2
+OPT_PANEL = 3
+OPT_FLOW_NEAR = 8
+OPT_BLAS = 7
+OPT_GEMM_KERNEL = 9
+
+# (name, option, candidates); the first candidate is the library default
+FACTOR_SPACE = [("panel", OPT_PANEL, [2, 1]), ("near", OPT_FLOW_NEAR, [8, 4, 2]), ("update", OPT_TILE_UPDATE, [64, 128])]
+SELINV_SPACE = [("grouped", OPT_TILE_GROUPED, [64, 128]), ("gemm_kernel", OPT_GEMM_KERNEL, [0, 1]),
+                ("blas", OPT_BLAS, [1, 0])]
+DEFAULTS = {"panel": 0, "near": 8, "update": 0, "grouped": 0, "gemm_kernel": 0, "blas": 1}
+OPTION = {name: opt for name, opt, _ in FACTOR_SPACE + SELINV_SPACE}
 
 _cache: dict = {}
 
@@ -47,17 +70,18 @@ def _time(fn, reps):
 
 
 def apply(solver, choice: dict) -> None:
+    """Set the workspace options of ``choice`` (missing names: library defaults)."""
     lib = _lib.load()
-    _lib.check(lib.bta_ws_set_option(solver._ws, OPT_TILE_UPDATE, int(choice.get("update", 0))), "set_option")
-    _lib.check(lib.bta_ws_set_option(solver._ws, OPT_TILE_GROUPED, int(choice.get("grouped", 0))), "set_option")
+    for name, opt in OPTION.items():
+        _lib.check(lib.bta_ws_set_option(solver._ws, opt, int(choice.get(name, DEFAULTS[name]))), "set_option")
 
 
-def autotune(solver, sparsity: str = "bta", candidates=(64, 128), reps: int = 2, cache_path: str | None = None,
-             selected_inversion: bool = True) -> dict:
-    """Pick the tile edge per GEMM class for ``solver``'s shape.
+def autotune(solver, sparsity: str = "bta", reps: int = 2, cache_path: str | None = None,
+             selected_inversion: bool = True, factor_space=None, selinv_space=None) -> dict:
+    """Pick the launch options for ``solver``'s shape.
 
     The solver's buffers must hold the (unfactored) matrix; they hold it again
-    on return.  Returns {"update": edge, "grouped": edge, "times_ms": {...}}.
+    on return.  Returns {option name: value, ..., "times_ms": {candidate: ms}}.
     """
     key = _key(solver, sparsity)
     if cache_path and os.path.exists(cache_path) and not _cache:
@@ -66,42 +90,47 @@ def autotune(solver, sparsity: str = "bta", candidates=(64, 128), reps: int = 2,
     if key in _cache:
         apply(solver, _cache[key])
         return _cache[key]
+    fspace = FACTOR_SPACE if factor_space is None else factor_space
+    sspace = SELINV_SPACE if selinv_space is None else selinv_space
+    if solver.diagonal_blocksize <= 64:  # one tile column: no persistent panel to tune
+        fspace = [d for d in fspace if d[0] not in ("panel", "near")]
     saved = [t.clone() for t in _blocks(solver)]
 
     def restore():
         for dst, src in zip(_blocks(solver), saved):
             dst.copy_(src)
 
-    times = {}
-    best_u, best_g = None, None
-    for tu in candidates:
-        apply(solver, {"update": tu, "grouped": 0})
-
-        def run_f():
-            solver.factorize(sparsity)
+    def run_f():
         restore()
-        run_f()  # warm (builds plans, captures graphs)
-        t = _time(lambda: (restore(), run_f()), reps)
-        times[f"update_{tu}"] = t
-        if best_u is None or t < times[f"update_{best_u}"]:
-            best_u = tu
-    if selected_inversion:
-        for tg in candidates:
-            apply(solver, {"update": best_u, "grouped": tg})
-            restore()
-            solver.factorize(sparsity)
-            solver.selected_inversion(sparsity)  # warm
-            torch.cuda.synchronize()
+        solver.factorize(sparsity)
 
-            def run_si():
-                restore()
-                solver.factorize(sparsity)
-                solver.selected_inversion(sparsity)
-            t = _time(run_si, reps)
-            times[f"grouped_{tg}"] = t
-            if best_g is None or t < times[f"grouped_{best_g}"]:
-                best_g = tg
-    choice = {"update": best_u, "grouped": best_g or 0, "times_ms": times}
+    def run_si():
+        restore()
+        solver.factorize(sparsity)
+        solver.selected_inversion(sparsity)
+
+    choice = dict(DEFAULTS)
+    times = {}
+
+    def descend(space, run):
+        for name, _opt, cands in space:
+            best_v, best_t = None, float("inf")
+            for v in cands:
+                trial = dict(choice)
+                trial[name] = v
+                apply(solver, trial)
+                run()  # warm: plans, schedules, graphs
+                torch.cuda.synchronize()
+                t = _time(run, reps)
+                times[f"{name}={v}"] = t
+                if t < best_t:
+                    best_v, best_t = v, t
+            choice[name] = best_v
+
+    descend(fspace, run_f)
+    if selected_inversion:
+        descend(sspace, run_si)
+    choice["times_ms"] = times
     apply(solver, choice)
     restore()
     _cache[key] = choice

@@ -363,52 +363,6 @@ __global__ void __launch_bounds__(BTA_THREADS, 4) k_gemm_grouped_db(const GemmJo
   } else {
     acc_zero(acc);
   }
-  // K range of segment sid for this tile (triangular operands skip zero chunks)
-  auto seg_range = [&](int sid, int& kb, int& ke) {
-    const GemmSeg& sg = g.seg[sid];
-    kb = 0;
-    ke = sg.k;
-    if (sg.tri & TRI_A_LOWER) ke = min(ke, (tr + 1) * 64);
-    if (sg.tri & TRI_A_UPPER) kb = max(kb, tr * 64);
-    if (sg.tri & TRI_B_LOWER) kb = max(kb, tc * 64);
-    if (sg.tri & TRI_B_UPPER) ke = min(ke, (tc + 1) * 64);
-  };
-  // chunk cursor: (segment, k0, end of the segment's range); seg == nseg: done
-  struct Cur { int seg, k0, ke; };
-  auto settle = [&](Cur c) {  // move to the first non-empty chunk at or after c
-    while (c.seg < g.nseg && c.k0 >= c.ke) {
-      ++c.seg;
-      if (c.seg < g.nseg) seg_range(c.seg, c.k0, c.ke);
-    }
-    return c;
-  };
-  auto first = [&]() {
-    Cur c{0, 0, 0};
-    if (g.nseg > 0) seg_range(0, c.k0, c.ke);
-    return settle(c);
-  };
-  auto next = [&](Cur c) {
-    c.k0 += G2_KC;
-    return settle(c);
-  };
-  auto fetch = [&](const Cur& c, G2Stage& st) {
-    const GemmSeg& sg = g.seg[c.seg];
-    const int kw = min(G2_KC, c.ke - c.k0);
-    const int64_t oa = sg.ta ? (int64_t)c.k0 * sg.lda + tr * 64 : (int64_t)(tr * 64) * sg.lda + c.k0;
-    const int64_t ob = sg.tb ? (int64_t)(tc * 64) * sg.ldb + c.k0 : (int64_t)c.k0 * sg.ldb + tc * 64;
-    st.a.load(sg.A + oa, sg.lda, sg.ta, rows, kw);
-    st.b.load(sg.B + ob, sg.ldb, 1 - sg.tb, cols, kw);
-    st.seg = c.seg;
-    st.k0 = c.k0;
-    st.kw = kw;
-  };
-  auto stage = [&](const G2Stage& st, int buf) {
-    const GemmSeg& sg = g.seg[st.seg];
-    const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
-    const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
-    st.a.store_a(sA[buf], sg.ta, sg.alpha, rows, st.kw, ma, tr * 64, st.k0);
-    st.b.store_b(sB[buf], sg.tb == 1 ? 0 : 1, cols, st.kw, mb, tc * 64, st.k0);
-  };
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
@@ -426,42 +380,52 @@ __global__ void __launch_bounds__(BTA_THREADS, 4) k_gemm_grouped_db(const GemmJo
       acc.c[1][1] = mfma_f64(a1, b1, acc.c[1][1]);
     }
   };
-  // number of chunks of the tile (all segments)
-  int nch = 0;
+  // One pipeline per segment, its fields read once: the K loop then touches
+  // no memory but the operands (segments are few, so refilling the pipeline
+  // at a segment boundary costs little).
   for (int sid = 0; sid < g.nseg; ++sid) {
-    int kb, ke;
-    seg_range(sid, kb, ke);
-    if (ke > kb) nch += (ke - kb + G2_KC - 1) / G2_KC;
-  }
-  if (nch > 0) {
-    // Chunk t sits in LDS buffer t % 2 when it is multiplied; past the last
-    // chunk the cursor stays on the first one (loaded, never multiplied), so
-    // every pass issues the same loads and stores and only the MFMAs of a
-    // missing odd chunk are skipped.
-    const Cur c0 = first();
-    G2Stage s0, s1;
-    Cur cf = c0;  // last chunk fetched (c0 again once past the end)
-    auto advance = [&]() {
-      const Cur c = next(cf);
-      cf = (c.seg < g.nseg) ? c : c0;
+    const GemmSeg sg = g.seg[sid];
+    int kb = 0, ke = sg.k;  // K range of this tile (triangular operands skip zero chunks)
+    if (sg.tri & TRI_A_LOWER) ke = min(ke, (tr + 1) * 64);
+    if (sg.tri & TRI_A_UPPER) kb = max(kb, tr * 64);
+    if (sg.tri & TRI_B_LOWER) kb = max(kb, tc * 64);
+    if (sg.tri & TRI_B_UPPER) ke = min(ke, (tc + 1) * 64);
+    if (ke <= kb) continue;
+    const int nch = (ke - kb + G2_KC - 1) / G2_KC;
+    const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
+    const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
+    // chunk q (past the last one: the last one again, loaded and never used)
+    auto fetch = [&](int q, G2Stage& st) {
+      const int c0 = kb + G2_KC * min(q, nch - 1);
+      const int kw = min(G2_KC, ke - c0);
+      const int64_t oa = sg.ta ? (int64_t)c0 * sg.lda + tr * 64 : (int64_t)(tr * 64) * sg.lda + c0;
+      const int64_t ob = sg.tb ? (int64_t)(tc * 64) * sg.ldb + c0 : (int64_t)c0 * sg.ldb + tc * 64;
+      st.a.load(sg.A + oa, sg.lda, sg.ta, rows, kw);
+      st.b.load(sg.B + ob, sg.ldb, 1 - sg.tb, cols, kw);
+      st.k0 = c0;
+      st.kw = kw;
     };
-    fetch(cf, s0);
-    advance();
-    fetch(cf, s1);
+    auto stage = [&](const G2Stage& st, int buf) {
+      st.a.store_a(sA[buf], sg.ta, sg.alpha, rows, st.kw, ma, tr * 64, st.k0);
+      st.b.store_b(sB[buf], 1 - sg.tb, cols, st.kw, mb, tc * 64, st.k0);
+    };
+    // Chunk t sits in LDS buffer t % 2 when it is multiplied; every pass
+    // issues the same loads and stores, only the MFMAs of a missing odd
+    // chunk are skipped.
+    G2Stage s0, s1;
+    fetch(0, s0);
+    fetch(1, s1);
     stage(s0, 0);
-    advance();
-    fetch(cf, s0);
+    fetch(2, s0);
     __syncthreads();
     for (int t = 0; t < nch; t += 2) {
-      mma(0);          // chunk t
-      stage(s1, 1);    // chunk t + 1
-      advance();
-      fetch(cf, s1);   // chunk t + 3
+      mma(0);           // chunk t
+      stage(s1, 1);     // chunk t + 1
+      fetch(t + 3, s1);
       lds_barrier();
       if (t + 1 < nch) mma(1);
-      stage(s0, 0);    // chunk t + 2
-      advance();
-      fetch(cf, s0);   // chunk t + 4
+      stage(s0, 0);     // chunk t + 2
+      fetch(t + 4, s0);
       lds_barrier();
     }
   }

@@ -535,6 +535,20 @@ class STGaussianModel:
         """(epoch, theta) of the factors the last forward_keep left in HBM."""
         return None if getattr(self, "_saved", None) is None else (self._saved[0], self._saved[1].copy())
 
+    def autotune(self, theta=None, reps: int = 2, cache_path: str | None = None) -> dict:
+        """Tune the launch options of both solvers (pyinla_amd.autotune) on
+        Q_prior(theta) and Q_cond(theta); the choices stay set on the
+        workspaces.  Returns {"prior": choice, "cond": choice}."""
+        from . import autotune as at
+        theta = np.asarray(self.prob.theta if theta is None else theta, dtype=np.float64)
+        self._saved = None
+        self.assemble_prior(theta)
+        cp = at.autotune(self.prior, "bt", reps=reps, cache_path=cache_path)
+        self.assemble_cond(theta)
+        cc = at.autotune(self.cond, "bta", reps=reps, cache_path=cache_path)
+        torch.cuda.synchronize(self.device)
+        return {"prior": cp, "cond": cc}
+
     def value_and_grad(self, theta):
         """f(theta) and its adjoint gradient: forward_keep then backward_grad."""
         parts = self.forward_keep(theta)

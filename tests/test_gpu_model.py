@@ -90,6 +90,25 @@ def test_adjoint_gradient_vs_dense_autograd_and_fd(cuda, small):
     assert _rel(g_fdg, g_fdo) < 1e-5
 
 
+def test_model_autotune_keeps_f_and_gradient(cuda, small):
+    """STGaussianModel.autotune times the launch options of both solvers on
+    Q_prior / Q_cond (panel variant, near pool, GEMM tiles and main loop,
+    rocBLAS or grouped products) and leaves the best set; f and the adjoint
+    gradient under the chosen options equal those under the defaults to the
+    FP64 bars."""
+    from pyinla_amd import autotune as at
+    at._cache.clear()
+    model = STGaussianModel(small, device=cuda)
+    f0, g0 = model.value_and_grad(small.theta)
+    ch = model.autotune(reps=1)
+    for c in ch.values():
+        assert set(at.DEFAULTS) <= set(c) and len(c["times_ms"]) >= 6
+    f1, g1 = model.value_and_grad(small.theta)
+    at._cache.clear()
+    assert abs(f1 - f0) <= 1e-10 * abs(f0)
+    assert np.abs(g1 - g0).max() <= 1e-8 * np.abs(g0).max()
+
+
 def test_adjoint_autograd_function(cuda, small):
     """AdjointF.forward runs only the forward (both factors stay in HBM, no
     selected inversion); backward runs the selected inversions in place and

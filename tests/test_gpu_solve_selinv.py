@@ -124,15 +124,19 @@ def test_batched_factorization_mixed_shapes(cuda):
 
 
 def test_autotuner_keeps_results_exact(cuda):
-    """The autotuner times both tile edges on the solver's matrix, restores the
-    blocks, and the factorization / selected inversion with the chosen tiles
-    still match the oracle."""
+    """The autotuner times its candidates (panel variant, near pool, GEMM
+    tiles, main loop, rocBLAS / grouped products) on the solver's matrix,
+    restores the blocks, and the factorization / selected inversion with the
+    chosen options still match the oracle."""
     from pyinla_amd import autotune
     kf = known_factor_bta(3, 600, 5, seed=77)
     s = SerinvSolver(None, 600, 3, 5)
     s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B, kf.Q_T)
+    autotune._cache.clear()
     choice = autotune.autotune(s, "bta", reps=1)
     assert choice["update"] in (64, 128) and choice["grouped"] in (64, 128)
+    assert choice["panel"] in (1, 2) and choice["near"] in (8, 4, 2)
+    assert choice["gemm_kernel"] in (0, 1) and choice["blas"] in (0, 1)
     assert _rel(s.A_diagonal_blocks.cpu().numpy(), kf.Q_D) < 1e-15  # restored
     s.factorize("bta")
     assert _rel(s.A_diagonal_blocks.cpu().numpy(), kf.Lf_D) < RTOL
@@ -210,6 +214,32 @@ def test_selected_inversion_blas_and_grouped(cuda, n, b, a, blas, use_graph):
         if a:
             assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
             assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
+
+
+@pytest.mark.parametrize("blas", [False, True])
+@pytest.mark.parametrize("n,b,a", [(2, 70, 2), (3, 130, 0), (3, 300, 6), (2, 600, 5)])
+def test_selected_inversion_double_buffered_gemm(cuda, n, b, a, blas):
+    """The double-buffered 64x64 grouped GEMM (BTA_OPT_GEMM_KERNEL 1: paired
+    clamped loads, masks at the LDS store, one pipeline per K segment) on the
+    block inverses and the selected inversion -- several segments per job,
+    triangular operands, ragged edges, lower / mirrored outputs -- against
+    the oracle."""
+    kf = known_factor_bta(n, b, a, seed=500 + n * b + a)
+    s = SerinvSolver(None, b, n, a, use_graph=False)
+    _lib.check(s.lib.bta_ws_set_option(s._ws, 9, 1), "set_option")
+    s.set_selinv_blas(blas)
+    D, L = kf.Lf_D.copy(), kf.Lf_L.copy()
+    B = kf.Lf_B.copy() if a else None
+    T = kf.Lf_T.copy() if a else None
+    ob.pobtasi(D, L, B, T)
+    s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if a else None, kf.Q_T if a else None)
+    s.factorize("bta")
+    s.selected_inversion("bta")
+    assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
+    assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < RTOL
+    if a:
+        assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
+        assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
 
 
 @pytest.mark.parametrize("n,b,a", [(3, 600, 5), (2, 130, 0), (4, 92, 4), (2, 300, 6)])
