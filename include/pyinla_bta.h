@@ -182,7 +182,8 @@ int bta_logdet(bta_ws_t ws, const double* D, const double* T, int n, int b, int 
 #define BTA_OPT_BLAS 7
 /* Persistent panel: the near-diagonal tiles ((c,c), (c+1,c), (c+2,c)), whose
  * updates and solves the chain waits for, go to a pool of helpers / value
- * helpers that own nothing else (default 8; 1..64). */
+ * helpers (at least min(10, helpers / 8)) that own nothing else (default 16;
+ * 1..64). */
 #define BTA_OPT_FLOW_NEAR 8
 /* 64x64 grouped GEMM main loop: 0 (default) = one LDS buffer, the next K
  * chunk loaded into registers while the current one is multiplied; 1 = two

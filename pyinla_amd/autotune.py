@@ -36,10 +36,10 @@ OPT_BLAS = 7
 OPT_GEMM_KERNEL = 9
 
 # (name, option, candidates); the first candidate is the library default
-FACTOR_SPACE = [("panel", OPT_PANEL, [2, 1]), ("near", OPT_FLOW_NEAR, [8, 4, 2]), ("update", OPT_TILE_UPDATE, [64, 128])]
+FACTOR_SPACE = [("panel", OPT_PANEL, [2, 1]), ("near", OPT_FLOW_NEAR, [16, 8, 24]), ("update", OPT_TILE_UPDATE, [64, 128])]
 SELINV_SPACE = [("grouped", OPT_TILE_GROUPED, [64, 128]), ("gemm_kernel", OPT_GEMM_KERNEL, [0, 1]),
                 ("blas", OPT_BLAS, [1, 0])]
-DEFAULTS = {"panel": 0, "near": 8, "update": 0, "grouped": 0, "gemm_kernel": 0, "blas": 1}
+DEFAULTS = {"panel": 0, "near": 16, "update": 0, "grouped": 0, "gemm_kernel": 0, "blas": 1}
 OPTION = {name: opt for name, opt, _ in FACTOR_SPACE + SELINV_SPACE}
 
 _cache: dict = {}

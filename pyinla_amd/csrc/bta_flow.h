@@ -79,7 +79,7 @@ __host__ __device__ inline void flow_decode(uint32_t w, int& op, int& rho, int& 
 // arrow tile (0, c) of B_{i+1}; rho == nT + 1 is the tip T (c = 0).
 struct FlowShape {
   int nT, nTL, nTa;
-  int near_div = 8;  // the near pool is helpers / near_div helpers (at most 3 nT)
+  int near_div = 16;  // near pool: helpers / near_div, at least min(10, helpers / 8), at most 3 nT
   int rows() const { return nT + nTL + nTa; }
 };
 
@@ -127,10 +127,17 @@ struct FlowArgs {
 // arguments are known to the compiler to be global (read from a table in
 // memory they would be generic and every access a FLAT one).
 #define FLOW_BATCH_MAX 8
+// Workgroup layout of a batched launch.  iso == 0: entry q owns workgroups
+// [begin[q], begin[q+1]), its chain first.  iso == 1 (two workgroups per CU,
+// workgroup p sharing its CU with workgroup p + cus): the chains are
+// workgroups 0 .. nent-1, their CU partners cus .. cus+nent-1 stay idle, so
+// no helper slows a chain; the other workgroups, in order, are the helpers,
+// entry q owning helper numbers [begin[q], begin[q+1]).
 struct FlowBatch {
   FlowArgs e[FLOW_BATCH_MAX];
-  int begin[FLOW_BATCH_MAX + 1];  // entry q owns workgroups [begin[q], begin[q+1])
+  int begin[FLOW_BATCH_MAX + 1];
   int nent;
+  int iso, cus;
 };
 
 int flow_owner(const FlowShape& s, int rho, int c, int helpers);

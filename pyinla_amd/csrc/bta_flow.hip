@@ -30,10 +30,16 @@ namespace bta {
 // them owns tiles spread evenly over the columns and its share of every
 // column's updates stays balanced; the next-block updates go to the same
 // pool.  Cleared upper tiles of D go by row-major index.
+// Pool size: helpers / near_div, but at least min(10, helpers / 8).  Measured
+// on MI355X (factorization pair on two streams, 255 helpers each): 31 near
+// helpers 1.42 ms per block pair at b = 2000 and 15.8 ms at b = 4500; 15-21
+// helpers 1.34 / 15.3; 10 helpers 1.33 / 15.0; 7 helpers 1.65 (too few: the
+// chain waits on them).  The floor keeps about 10 for the smaller helper
+// counts of batched launches.
 static int flow_near_pool(const FlowShape& s, int helpers) {
-  const int div = s.near_div > 0 ? s.near_div : 8;
+  const int div = s.near_div > 0 ? s.near_div : 16;
   if (helpers < 8) return 0;
-  return std::min(3 * s.nT, helpers / div);
+  return std::min(3 * s.nT, std::max(helpers / div, std::min(10, helpers / 8)));
 }
 
 static bool flow_is_near(int nT, int rho, int c) { return rho < nT && rho >= c && rho - c <= 2; }
@@ -885,22 +891,34 @@ void flow_launch_block(const Workspace& ws, const FlowPlan& fp, const FlowSched&
   hipLaunchKernelGGL(k_panel_flow, dim3(1 + sc.helpers), dim3(BTA_THREADS), 0, st, p);
 }
 
-// Batched form: entry e of a launch owns workgroups [begin[e], begin[e+1]):
-// its chain first, then its helpers.  All entries' workgroups are resident
-// together (their sum is within the device's capacity).
+// Batched form: every entry's chain and helpers in one launch (layout in
+// FlowBatch).  All entries' workgroups are resident together (their sum is
+// within the device's capacity).
 __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow_batched(FlowBatch fb) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
   __shared__ double sScr[4 * 64 + 64];
   __shared__ int ok_sh;
-  int lo = 0, hi = fb.nent - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (fb.begin[mid] <= (int)blockIdx.x) lo = mid;
-    else hi = mid - 1;
+  const int wg = (int)blockIdx.x;
+  int pos = wg;  // position in the entries' ranges of `begin`
+  if (fb.iso) {
+    if (wg >= fb.cus && wg < fb.cus + fb.nent) return;  // the CU partner of a chain
+    if (wg < fb.nent) pos = -1;                         // chain of entry wg
+    else pos = wg < fb.cus ? wg - fb.nent : wg - 2 * fb.nent;
+  }
+  int lo = 0, local = 0;
+  if (pos < 0) {
+    lo = wg;
+  } else {
+    int hi = fb.nent - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (fb.begin[mid] <= pos) lo = mid;
+      else hi = mid - 1;
+    }
+    local = pos - fb.begin[lo] + (fb.iso ? 1 : 0);
   }
   const FlowArgs p = fb.e[lo];
-  const int local = (int)blockIdx.x - fb.begin[lo];
   const unsigned long long tag = (p.epoch[0] << 20) | (unsigned long long)p.blk;
   if (local == 0) flow_chain(p, tag, sA, sB, sScr, &ok_sh);
   else flow_helper(p, local - 1, tag, sA, sB, &ok_sh);

@@ -41,7 +41,7 @@ struct FlowSched {
   uint32_t* d_tasks = nullptr;
   int* d_off = nullptr;
   int off_base[2] = {0, 0};
-  int helpers = 0, delta = 0, nTa = 0, near_div = 8;
+  int helpers = 0, delta = 0, nTa = 0, near_div = 16;
   size_t bytes = 0;
 };
 
@@ -161,7 +161,7 @@ struct Workspace {
   int panel_mode = 0;
   int flow_helpers = 0;  // 0 = automatic
   int flow_delta = 0;    // 0: list-scheduled task order; >= 1: level order, far-tile update groups
-  int flow_near = 8;     // near pool = helpers / flow_near
+  int flow_near = 16;    // near pool = helpers / flow_near (see flow_near_pool)
   int gemm_variant = 0;  // 64x64 grouped GEMM: 0 = single LDS buffer, 1 = two LDS buffers
   int batch_lanes = 0;   // problems per wave of a batched factorization (0 = 6)
   // persistent-panel exchange buffers, sized for the workspace's arrowhead

@@ -601,6 +601,8 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
     // helpers per problem: the device's resident capacity split evenly over
     // the problems still active at each block step
     const int cap = flow_capacity(device);
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 0;
     const int np = (int)ps.size();
     const int nl = std::min(lanes > 0 ? std::min(lanes, np) : std::min(np, 6), FLOW_BATCH_MAX);
     if (cap / nl - 1 < 1) {
@@ -638,9 +640,16 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
         for (int i = 0; i < wmax; ++i) {
           int nact = 0;
           for (int t = w0; t < w1; ++t) nact += (i < ps[order[t]].n) ? 1 : 0;
-          const int helpers = std::min(1023, cap / nact - 1);
+          // two workgroups per CU: each chain gets a CU of its own (its
+          // partner slot idle), measured on MI355X to shorten the chain's
+          // steps as much as a co-resident helper had lengthened them
+          const int h_iso = (cap - 2 * nact) / nact;
+          const bool iso = cus > 0 && cap == 2 * cus && h_iso >= 1 && nact * (2 + h_iso) >= cus + nact;
+          const int helpers = std::min(1023, iso ? h_iso : cap / nact - 1);
           FLaunch L;
           L.fb.nent = 0;
+          L.fb.iso = iso ? 1 : 0;
+          L.fb.cus = cus;
           int total = 0;
           for (int t = w0; t < w1; ++t) {
             const int q = order[t];
@@ -651,11 +660,13 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
             if (rc) return rc;
             L.fb.e[L.fb.nent] = flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch,
                                           d.info, d.err, i, d.n, d.b, d.a);
-            L.fb.begin[L.fb.nent++] = total;
+            const int j = L.fb.nent;
+            L.fb.begin[j] = iso ? j * helpers : total;  // iso: helper numbers
+            L.fb.nent = j + 1;
             total += 1 + helpers;
           }
-          L.fb.begin[L.fb.nent] = total;
-          L.total = total;
+          L.fb.begin[L.fb.nent] = iso ? L.fb.nent * helpers : total;
+          L.total = iso ? total + L.fb.nent : total;  // + the chains' idle partners
           flaunches.push_back(L);
         }
       }
