@@ -148,10 +148,10 @@ static void flow_build_levels(const FlowShape& s, int helpers, int delta, FlowSc
 // all workgroups resident the earliest unfinished task in that order can
 // always run, whatever the real durations: no interleaving can deadlock.
 #ifndef FLOW_COST_CHOL
-#define FLOW_COST_CHOL 38600  // chain: factor + inverse of a diagonal tile, stores
+#define FLOW_COST_CHOL 30000  // chain: factor + inverse of a diagonal tile, stores, publish
 #endif
 #ifndef FLOW_COST_CTRSM
-#define FLOW_COST_CTRSM 11400  // chain: solve of the sub-diagonal tile, stores, publish
+#define FLOW_COST_CTRSM 10000  // chain: solve of the sub-diagonal tile, stores
 #endif
 #ifndef FLOW_COST_CUPD
 #define FLOW_COST_CUPD 5300  // chain: update of the next diagonal tile
@@ -236,10 +236,8 @@ static bool flow_build_list(const FlowShape& s, int helpers, FlowSchedule& out) 
       if (!diag && !(rho < nT && rho == c + 1)) {
         const int t = add(owner, FLOW_COST_TRSM, FLOW_TRSM, flow_encode(FLOW_TRSM, rho, c, c), tid);
         edge(prev, t);
-        // inv(L_cc) is published when the chain solves the sub-diagonal tile
-        // of step c (after its waits for the step's hand-off tiles), or right
-        // after the factorization for the last column
-        edge(c + 1 < nT ? ct[c] : cf[c], t);
+        // inv(L_cc) is published right after the chain's factorization
+        edge(cf[c], t);
         xprod[tid] = t;
       }
     }
@@ -528,14 +526,14 @@ __device__ __forceinline__ void chain_store_diag(const FlowArgs& p, int k, const
 // Per column k the chain
 //   1. waits for the hand-off tiles (k+1, k) and (k+1, k+1) (updated by their
 //      helpers through column k-1) and loads both;
-//   2. solves X = C(k+1, k) inv(L_kk)^T and issues its stores;
-//   3. publishes inv(L_kk) (stored one step earlier: the stores have drained);
-//   4. applies C(k+1, k+1) -= X X^T, then publishes X (its stores drained
+//   2. solves X = C(k+1, k) inv(L_kk)^T and issues the stores of X;
+//   3. applies C(k+1, k+1) -= X X^T, then publishes X (its stores drained
 //      during the update);
-//   5. factors C(k+1, k+1) and issues the stores of L and its inverse.
-// Deferring each flag past the next piece of work delays its consumers by a
-// few microseconds; they are needed one or two columns later.
-__device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA, double* sB, double* sScr,
+//   4. factors C(k+1, k+1), stores L and its inverse and publishes the
+//      inverse at once: the helpers' solves of column k+1 wait for it, and the
+//      stores drain while they are issued (a few hundred cycles of wait,
+//      measured on MI355X).
+__device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA, double* sB,
                            int* ok_sh) {
   const int nT = p.nT;
   int rows, cols;
@@ -545,9 +543,9 @@ __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA
     acc_load(acc, g, p.b, rows, cols);
     acc_to_lds(acc, sA);
     __syncthreads();
-    const TileCholResult res = tile_chol_inv(sA, sB, sScr, rows);
+    const TileCholResult res = tile_chol_inv(sA, sB, rows);
     chain_store_diag(p, 0, res, sA, sB);
-    if (nT == 1) flow_publish(wflag(p, 0), tag);
+    flow_publish(wflag(p, 0), tag);
   }
   for (int k = 0; k + 1 < nT; ++k) {
     if (k == 0) {
@@ -562,22 +560,22 @@ __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA
     acc_to_lds(acc, sA);
     __syncthreads();
     flow_trsm(acc, sA, sB);  // X(k+1, k) in acc
-    __syncthreads();
+    lds_barrier();           // the reads of sA are done
     {
       double* g = flow_tile(p, k + 1, k, rows, cols);
       acc_store(acc, g, p.b, rows, cols);
       acc_store_sys(acc, xtile(p, k + 1, k));
       acc_to_lds(acc, sA);
     }
-    flow_publish(wflag(p, k), tag);  // inv(L_kk), stored during the previous step
+    lds_barrier();
     mma_nt<true>(accd, sA, sA, 0, 64);
     flow_publish(xflag(p, k + 1, k), tag);
     acc_to_lds(accd, sA);
     __syncthreads();
     const int mv = min(64, p.b - 64 * (k + 1));
-    const TileCholResult res = tile_chol_inv(sA, sB, sScr, mv);
+    const TileCholResult res = tile_chol_inv(sA, sB, mv);
     chain_store_diag(p, k + 1, res, sA, sB);
-    if (k + 2 == nT) flow_publish(wflag(p, k + 1), tag);
+    flow_publish(wflag(p, k + 1), tag);
   }
 }
 
@@ -755,10 +753,9 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
 __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow(FlowArgs p) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ double sScr[4 * 64 + 64];
   __shared__ int ok_sh;
   const unsigned long long tag = (p.epoch[0] << 20) | (unsigned long long)p.blk;
-  if (blockIdx.x == 0) flow_chain(p, tag, sA, sB, sScr, &ok_sh);
+  if (blockIdx.x == 0) flow_chain(p, tag, sA, sB, &ok_sh);
   else flow_helper(p, blockIdx.x - 1, tag, sA, sB, &ok_sh);
 }
 
@@ -897,7 +894,6 @@ void flow_launch_block(const Workspace& ws, const FlowPlan& fp, const FlowSched&
 __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow_batched(FlowBatch fb) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ double sScr[4 * 64 + 64];
   __shared__ int ok_sh;
   const int wg = (int)blockIdx.x;
   int pos = wg;  // position in the entries' ranges of `begin`
@@ -920,7 +916,7 @@ __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow_batched(FlowBatch
   }
   const FlowArgs p = fb.e[lo];
   const unsigned long long tag = (p.epoch[0] << 20) | (unsigned long long)p.blk;
-  if (local == 0) flow_chain(p, tag, sA, sB, sScr, &ok_sh);
+  if (local == 0) flow_chain(p, tag, sA, sB, &ok_sh);
   else flow_helper(p, local - 1, tag, sA, sB, &ok_sh);
 }
 

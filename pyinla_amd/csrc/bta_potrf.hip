@@ -102,7 +102,7 @@ __host__ __device__ inline int panel_grid(int nT, int nTL, int nTa, int k) {
 }
 
 __device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* __restrict__ sA,
-                                           double* __restrict__ sB, double* __restrict__ sScr) {
+                                           double* __restrict__ sB) {
   const TileRef t = decode_panel_tile(p, idx);
   const unsigned long long ep = (*p.epoch << 20) | p.blk;
   const int b = p.b, k = p.k;
@@ -147,7 +147,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* 
     acc_to_lds(acc, sA);
     __syncthreads();
     const int mv = rows;  // == cols
-    const TileCholResult res = tile_chol_inv(sA, sB, sScr, mv);
+    const TileCholResult res = tile_chol_inv(sA, sB, mv);
     // store L_kk (full tile: zero strict upper) and inv(L_kk)
     {
       const int c = threadIdx.x & 63;
@@ -188,8 +188,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* 
 __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_step(PanelArgs p) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ double sScr[4 * 64 + 64];
-  panel_body(p, blockIdx.x, sA, sB, sScr);
+  panel_body(p, blockIdx.x, sA, sB);
 }
 
 // Batched form: one launch covers tile column k of block step i of every
@@ -227,7 +226,6 @@ __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_step_batched(const Pro
                                                                    int i, int k) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ double sScr[4 * 64 + 64];
   int lo = 0, hi = nent - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -235,7 +233,7 @@ __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_step_batched(const Pro
     else hi = mid - 1;
   }
   const PanelArgs p = panel_args_of(probs[ent_prob[lo]], i, k);
-  panel_body(p, blockIdx.x - ent_begin[lo], sA, sB, sScr);
+  panel_body(p, blockIdx.x - ent_begin[lo], sA, sB);
 }
 
 __global__ void k_begin_batch(const ProbDesc* probs, int nprob) {
@@ -328,10 +326,9 @@ __global__ void __launch_bounds__(BTA_THREADS) k_tip_potrf(double* T, int a, dou
                                                           double* tinv) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ double sScr[4 * 64 + 64];
   lds_load_tile(sA, T, a, a, a);
   __syncthreads();
-  const TileCholResult res = tile_chol_inv(sA, sB, sScr, a);
+  const TileCholResult res = tile_chol_inv(sA, sB, a);
   const int c = threadIdx.x & 63;
   for (int r = threadIdx.x >> 6; r < 64; r += 4) {
     if (r < a && c < a) T[(int64_t)r * a + c] = sA[r * BTA_LDS_LD + c];
@@ -363,13 +360,12 @@ __global__ void __launch_bounds__(BTA_THREADS) k_logdet_reduce(const double* par
 __global__ void __launch_bounds__(BTA_THREADS) k_tip_potrf_batched(const ProbDesc* probs) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ double sScr[4 * 64 + 64];
   const ProbDesc& d = probs[blockIdx.x];
   const int a = d.a;
   if (a <= 0) return;
   lds_load_tile(sA, d.T, a, a, a);
   __syncthreads();
-  const TileCholResult res = tile_chol_inv(sA, sB, sScr, a);
+  const TileCholResult res = tile_chol_inv(sA, sB, a);
   const int c = threadIdx.x & 63;
   for (int r = threadIdx.x >> 6; r < 64; r += 4) {
     if (r < a && c < a) d.T[(int64_t)r * a + c] = sA[r * BTA_LDS_LD + c];

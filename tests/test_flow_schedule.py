@@ -110,15 +110,13 @@ class Sim:
     def chain(self):
         """The chain workgroup with the device's wait and publish points
         (flow_chain): per column k it waits for BOTH hand-off tiles before
-        the solve of (k+1, k); inv(L_kk) is published only then (stored one
-        step earlier), X(k+1, k) after the update of (k+1, k+1), and the last
-        inverse right after the last factorization."""
+        the solve of (k+1, k); X(k+1, k) is published after the update of
+        (k+1, k+1), and every inverse right after its factorization."""
         nT = self.nT
         Lf, W = self.chol(self.tile(0, 0), self.valid(0))
         self.store("chain", 0, 0, Lf)
         self.winv[0] = W
-        if nT == 1:
-            self.flags.add(("w", 0))
+        self.flags.add(("w", 0))
         for k in range(nT - 1):
             if k == 0:
                 Cs = self.tile(1, 0).copy()
@@ -133,15 +131,13 @@ class Sim:
             X = Cs @ W.T
             self.store("chain", k + 1, k, X)
             self.xbuf[(k + 1, k)] = X.copy()
-            self.flags.add(("w", k))
             Cd -= X @ X.T
             self.flags.add(("x", k + 1, k))
             self.nupd[(k + 1, k + 1)] = k + 1
             Lf, W = self.chol(Cd, self.valid(k + 1))
             self.store("chain", k + 1, k + 1, Lf)
             self.winv[k + 1] = W
-            if k + 2 == nT:
-                self.flags.add(("w", k + 1))
+            self.flags.add(("w", k + 1))
 
     def helper(self, h, tasks):
         nT = self.nT
@@ -325,8 +321,8 @@ def test_list_order_at_production_shape(nT, nTL, nTa, helpers, near):
     columns) with full-device helper counts, blocks with and without L: no
     interleaving deadlocks and the factor is right (small tiles keep the
     numerics cheap; the schedule depends on the tile counts only).  The last
-    block of the b = 4500 shape deadlocked on the device before TRSM tasks
-    depended on the chain step that publishes inv(L_kk)."""
+    block of the b = 4500 shape deadlocked on the device while TRSM tasks
+    did not depend on the chain step that publishes inv(L_kk)."""
     ts = 4 if nT <= 32 else 2
     b = nT * ts
     D, L, B = _panel(b, min(3, ts) if nTa else 0, nTL > 0, 5)
