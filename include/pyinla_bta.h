@@ -190,6 +190,12 @@ int bta_logdet(bta_ws_t ws, const double* D, const double* T, int n, int b, int 
  * LDS buffers with loads issued two K chunks ahead (slower on MI355X: 470 us
  * against 395 us for NN 2000^3; kept for comparison). */
 #define BTA_OPT_GEMM_KERNEL 9
+/* Persistent panel, block update D_{i+1} -= L_i L_i^T, B_{i+1} -= B_i L_i^T,
+ * T -= B_i B_i^T: 1 = applied by each block's launch through helper tasks
+ * that fill the gaps of the factorization; 2 = the launch only factors the
+ * panel and the update follows as one grouped-GEMM launch; 0 (default) =
+ * automatic (1 up to 48 tile columns, b <= 3072, else 2). */
+#define BTA_OPT_FLOW_FOLD 10
 int bta_ws_set_option(bta_ws_t ws, int option, int value);
 
 /* Host-only diagnostic (no reference counterpart): the helper task lists of

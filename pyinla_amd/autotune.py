@@ -5,8 +5,9 @@ speed but not results beyond rounding:
 
   factorization       panel variant (persistent dataflow panel, or one launch
                       per 64-wide tile column), the near-pool divisor of the
-                      persistent panel, the output tile of the block-update
-                      GEMMs of the per-column panel (64 / 128);
+                      persistent panel, whether it applies the block update
+                      itself or leaves it to a grouped GEMM, the output tile
+                      of the block-update GEMMs (64 / 128);
   selected inversion  output tile of the grouped GEMM (64 / 128), its 64x64
                       main loop (one or two LDS buffers), and whether the
                       Sigma_{i+1,i} products go to rocBLAS dgemm or stay in the
@@ -34,12 +35,14 @@ OPT_PANEL = 3
 OPT_FLOW_NEAR = 8
 OPT_BLAS = 7
 OPT_GEMM_KERNEL = 9
+OPT_FLOW_FOLD = 10
 
-# (name, option, candidates); the first candidate is the library default
-FACTOR_SPACE = [("panel", OPT_PANEL, [2, 1]), ("near", OPT_FLOW_NEAR, [16, 8, 24]), ("update", OPT_TILE_UPDATE, [64, 128])]
+# (name, option, candidates); DEFAULTS holds the library defaults
+FACTOR_SPACE = [("panel", OPT_PANEL, [2, 1]), ("near", OPT_FLOW_NEAR, [16, 8, 24]), ("fold", OPT_FLOW_FOLD, [1, 2]),
+                ("update", OPT_TILE_UPDATE, [64, 128])]
 SELINV_SPACE = [("grouped", OPT_TILE_GROUPED, [64, 128]), ("gemm_kernel", OPT_GEMM_KERNEL, [0, 1]),
                 ("blas", OPT_BLAS, [1, 0])]
-DEFAULTS = {"panel": 0, "near": 16, "update": 0, "grouped": 0, "gemm_kernel": 0, "blas": 1}
+DEFAULTS = {"panel": 0, "near": 16, "fold": 0, "update": 0, "grouped": 0, "gemm_kernel": 0, "blas": 1}
 OPTION = {name: opt for name, opt, _ in FACTOR_SPACE + SELINV_SPACE}
 
 _cache: dict = {}
@@ -93,7 +96,7 @@ def autotune(solver, sparsity: str = "bta", reps: int = 2, cache_path: str | Non
     fspace = FACTOR_SPACE if factor_space is None else factor_space
     sspace = SELINV_SPACE if selinv_space is None else selinv_space
     if solver.diagonal_blocksize <= 64:  # one tile column: no persistent panel to tune
-        fspace = [d for d in fspace if d[0] not in ("panel", "near")]
+        fspace = [d for d in fspace if d[0] not in ("panel", "near", "fold")]
     saved = [t.clone() for t in _blocks(solver)]
 
     def restore():

@@ -468,6 +468,7 @@ int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   if (option == BTA_OPT_BLAS && value != 0 && value != 1) return BTA_E_ARG;
   if (option == BTA_OPT_FLOW_NEAR && (value < 1 || value > 64)) return BTA_E_ARG;
   if (option == BTA_OPT_GEMM_KERNEL && value != 0 && value != 1) return BTA_E_ARG;
+  if (option == BTA_OPT_FLOW_FOLD && (value < 0 || value > 2)) return BTA_E_ARG;
   std::lock_guard<std::mutex> g(ws->mu);
   if (option == BTA_OPT_TILE_UPDATE) ws->w.tile_update = value;
   else if (option == BTA_OPT_TILE_GROUPED) ws->w.tile_grouped = value;
@@ -484,6 +485,7 @@ int bta_ws_set_option(bta_ws_t ws, int option, int value) {
   else if (option == BTA_OPT_BLAS) ws->w.use_blas = value;  // part of the plan key
   else if (option == BTA_OPT_FLOW_NEAR) ws->w.flow_near = value;
   else if (option == BTA_OPT_GEMM_KERNEL) ws->w.gemm_variant = value;
+  else if (option == BTA_OPT_FLOW_FOLD) ws->w.flow_fold = value;
   else return BTA_E_ARG;
   // recorded graphs embed the old launch shapes
   drop_graphs(ws);

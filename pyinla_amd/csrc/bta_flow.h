@@ -73,13 +73,14 @@ __host__ __device__ inline void flow_decode(uint32_t w, int& op, int& rho, int& 
 }
 
 // Shape of a panel: nT tile columns (and D tile rows), nTL L tile rows (0 for
-// the last block), nTa arrow tile rows (0 or 1).  With nTL > 0 the launch also
-// updates D_{i+1} (and B_{i+1}); with nTa > 0 it updates the tip T.
+// the last block), nTa arrow tile rows (0 or 1).  With fold set, nTL > 0 the
+// launch also updates D_{i+1} (and B_{i+1}); with nTa > 0 it updates the tip T.
 // FLOW_NEXT targets: row rho < nT is D_{i+1} tile (rho, c); rho == nT is the
 // arrow tile (0, c) of B_{i+1}; rho == nT + 1 is the tip T (c = 0).
 struct FlowShape {
   int nT, nTL, nTa;
   int near_div = 16;  // near pool: helpers / near_div, at least min(10, helpers / 8), at most 3 nT
+  int fold = 1;       // 0: no next-block tasks (the caller applies the block update as a GEMM)
   int rows() const { return nT + nTL + nTa; }
 };
 

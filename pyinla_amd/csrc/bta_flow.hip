@@ -94,13 +94,13 @@ static void flow_build_levels(const FlowShape& s, int helpers, int delta, FlowSc
       v.push_back({key, flow_encode(FLOW_NEXT, rho, c, k), flow_next_owner(s, rho, c, helpers)});
     }
   };
-  if (s.nTL > 0) {
+  if (s.fold && s.nTL > 0) {
     for (int r = 0; r < nT; ++r)
       for (int c = 0; c <= r; ++c) push_next(r, c);
     if (s.nTa > 0)
       for (int c = 0; c < nT; ++c) push_next(nT, c);
   }
-  if (s.nTa > 0) push_next(nT + 1, 0);
+  if (s.fold && s.nTa > 0) push_next(nT + 1, 0);
   for (int rho = 0; rho < R; ++rho) {
     for (int c = 0; c < nT; ++c) {
       if (rho < nT && rho < c) {  // strict upper tile of D: cleared
@@ -288,13 +288,13 @@ static bool flow_build_list(const FlowShape& s, int helpers, FlowSchedule& out) 
       if (rho != c) edge(xprod[(size_t)c * nT + g1], u.first);
     }
   }
-  if (s.nTL > 0) {
+  if (s.fold && s.nTL > 0) {
     for (int r = 0; r < nT; ++r)
       for (int c = 0; c <= r; ++c) push_next(r, c);
     if (s.nTa > 0)
       for (int c = 0; c < nT; ++c) push_next(nT, c);
   }
-  if (s.nTa > 0) push_next(nT + 1, 0);
+  if (s.fold && s.nTa > 0) push_next(nT + 1, 0);
   succ.resize(agent.size());
   // chain order and its waits on the hand-off tiles
   for (int k = 0; k < nT; ++k) {
@@ -800,10 +800,11 @@ int FlowPlan::build(int nT_, int nTa_) {
   return 0;
 }
 
-int FlowPlan::sched(int nTa_s, int helpers, int delta, int near_div, const FlowSched** out) {
+int FlowPlan::sched(int nTa_s, int helpers, int delta, int near_div, int fold, const FlowSched** out) {
   if (nTa_s > nTa) return (int)hipErrorInvalidValue;
   for (const auto& sc : scheds)
-    if (sc.helpers == helpers && sc.delta == delta && sc.nTa == nTa_s && sc.near_div == near_div) {
+    if (sc.helpers == helpers && sc.delta == delta && sc.nTa == nTa_s && sc.near_div == near_div &&
+        sc.fold == fold) {
       *out = &sc;
       return 0;
     }
@@ -812,10 +813,11 @@ int FlowPlan::sched(int nTa_s, int helpers, int delta, int near_div, const FlowS
   sc.delta = delta;
   sc.nTa = nTa_s;
   sc.near_div = near_div;
+  sc.fold = fold;
   std::vector<uint32_t> all;
   std::vector<int> off;
   for (int v = 0; v < 2; ++v) {  // 0: block with L, 1: last block (no L)
-    FlowShape s{nT, v == 0 ? nT : 0, nTa_s, near_div};
+    FlowShape s{nT, v == 0 ? nT : 0, nTa_s, near_div, fold};
     FlowSchedule h;
     if (!flow_build_schedule(s, helpers, delta, h)) return -1;
     const int base = (int)all.size();

@@ -41,7 +41,7 @@ struct FlowSched {
   uint32_t* d_tasks = nullptr;
   int* d_off = nullptr;
   int off_base[2] = {0, 0};
-  int helpers = 0, delta = 0, nTa = 0, near_div = 16;
+  int helpers = 0, delta = 0, nTa = 0, near_div = 16, fold = 1;
   size_t bytes = 0;
 };
 
@@ -67,7 +67,7 @@ struct FlowPlan {
   int build(int nT, int nTa);
   // schedule for (nTa, helpers, delta) with nTa <= the buffers' nTa, built on
   // first request (host; never during capture)
-  int sched(int nTa, int helpers, int delta, int near_div, const FlowSched** out);
+  int sched(int nTa, int helpers, int delta, int near_div, int fold, const FlowSched** out);
 };
 
 int flow_helpers_default(int device);
@@ -162,6 +162,7 @@ struct Workspace {
   int flow_helpers = 0;  // 0 = automatic
   int flow_delta = 0;    // 0: list-scheduled task order; >= 1: level order, far-tile update groups
   int flow_near = 16;    // near pool = helpers / flow_near (see flow_near_pool)
+  int flow_fold = 0;     // block update: 0 = automatic, 1 = in the persistent panel, 2 = a grouped GEMM after it
   int gemm_variant = 0;  // 64x64 grouped GEMM: 0 = single LDS buffer, 1 = two LDS buffers
   int batch_lanes = 0;   // problems per wave of a batched factorization (0 = 6)
   // persistent-panel exchange buffers, sized for the workspace's arrowhead

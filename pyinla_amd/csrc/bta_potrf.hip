@@ -468,6 +468,18 @@ int flow_alloc(Workspace& ws) {
   return 0;
 }
 
+// Whether the persistent panel applies the block update itself.  Measured on
+// MI355X (factorizations of Q_prior and Q_cond, ms per block, lone / pair on
+// two streams): b = 2000 (32 tile columns) in the panel 0.918 / 1.318, as a
+// GEMM after it 0.941 / 1.397; b = 4500 (71 tile columns) in the panel
+// 9.10 / 15.47, as a GEMM 7.46 / 14.16.  The helpers are the bound in both
+// cases: for wide panels the separate update (one large grouped GEMM) is the
+// cheaper way to do its share of the work.
+static int flow_fold_of(const Workspace& ws, int b) {
+  if (ws.flow_fold == 1 || ws.flow_fold == 2) return ws.flow_fold == 1;
+  return (b + 63) / 64 <= 48;
+}
+
 static int flow_prepare(Workspace& ws, int b, int a) {
   const int nT = (b + 63) / 64, nTa = (a + 63) / 64;
   int helpers = ws.flow_helpers > 0 ? ws.flow_helpers : flow_helpers_default(ws.device);
@@ -476,7 +488,7 @@ static int flow_prepare(Workspace& ws, int b, int a) {
   // the exchange buffers come with the workspace (bta_ws_create, or the
   // option call that selected the flow panel)
   if (!ws.flow || ws.flow->nT != nT || ws.flow->nTa < nTa) return (int)hipErrorNotReady;
-  return ws.flow->sched(nTa, helpers, ws.flow_delta, ws.flow_near, &ws.flow_sched);
+  return ws.flow->sched(nTa, helpers, ws.flow_delta, ws.flow_near, flow_fold_of(ws, b), &ws.flow_sched);
 }
 
 int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int n, int b, int a) {
@@ -485,13 +497,17 @@ int potrf_prepare(Workspace& ws, double* D, double* L, double* B, double* T, int
     if (rc) return rc;
   }
   ws.update_cur = nullptr;
-  if (choose_tile(ws.tile_update, b, b) != 128) return 0;
-  const PlanKey key{{D, L, B, T}, a, 128, 0};
+  // grouped-GEMM block updates: with 128-wide tiles, or after a persistent
+  // panel that does not apply them itself
+  const bool sep = use_flow(ws, b) && !flow_fold_of(ws, b);
+  const int tsu = choose_tile(ws.tile_update, b, b);
+  if (tsu != 128 && !sep) return 0;
+  const PlanKey key{{D, L, B, T}, a, tsu, 0};
   int rc = 0;
   ws.update_cur = plan_lookup(ws, ws.update_plans, key, [&](DevPlan& plan) {
     std::vector<GemmLaunch> ls;
     for (int i = 0; i < n; ++i) {
-      GemmLaunch g(128);
+      GemmLaunch g(tsu);
       double* Li = (i < n - 1) ? L + (int64_t)i * b * b : nullptr;
       double* Bi = (a > 0) ? B + (int64_t)i * a * b : nullptr;
       if (i < n - 1) {
@@ -543,10 +559,12 @@ int potrf_enqueue(Workspace& ws, double* D, double* L, double* B, double* T, int
     p.nT = nT;
     p.nTL = p.L ? nT : 0;
     p.nTa = p.B ? nTa : 0;
-    if (flow) {  // the launch also applies the block update
-      TimedScope ts(ws, KC_PANEL, st);
-      flow_launch_block(ws, *ws.flow, *ws.flow_sched, D, L, B, T, i, n, b, a, st);
-      continue;
+    if (flow) {
+      {
+        TimedScope ts(ws, KC_PANEL, st);
+        flow_launch_block(ws, *ws.flow, *ws.flow_sched, D, L, B, T, i, n, b, a, st);
+      }
+      if (ws.flow_sched->fold) continue;  // the launch also applied the block update
     } else {
       for (int k = 0; k < nT; ++k) {
         p.k = k;
@@ -652,7 +670,7 @@ int BatchPlan::build(const std::vector<ProbDesc>& ps, int panel_mode, int lanes,
             const ProbDesc& d = ps[q];
             if (i >= d.n) continue;
             const FlowSched* sc = nullptr;
-            const int rc = fplans[q]->sched(d.a > 0 ? d.nTa : 0, helpers, delta, near_div, &sc);
+            const int rc = fplans[q]->sched(d.a > 0 ? d.nTa : 0, helpers, delta, near_div, 1, &sc);
             if (rc) return rc;
             L.fb.e[L.fb.nent] = flow_args(*fplans[q], *sc, d.D, d.L, d.B, d.T, d.winv, d.logpart, d.epoch,
                                           d.info, d.err, i, d.n, d.b, d.a);

@@ -121,20 +121,29 @@ class SerinvSolver:
                                           _stream_ptr(self.device)), "bta_potrf")
         self._factored = sparsity
 
-    def set_panel(self, mode: str = "auto", helpers: int = 0, delta: int | None = None) -> None:
+    def set_panel(self, mode: str = "auto", helpers: int = 0, delta: int | None = None,
+                  fold: str | None = None) -> None:
         """Panel factorization variant of ``factorize`` (no reference
         counterpart): "auto", "columns" (one launch per 64-wide tile column)
         or "flow" (one persistent dataflow launch per block); ``helpers`` sets
         the flow launch's helper workgroups (0 = automatic) and ``delta`` the
         order of their task lists (0 = simulated critical-path order, d >= 1 =
-        dependency levels with far-tile updates grouped by d columns)."""
+        dependency levels with far-tile updates grouped by d columns);
+        ``fold`` whether the flow launch applies the block update itself
+        ("panel"), leaves it to a grouped GEMM after it ("gemm") or picks by
+        the block size ("auto")."""
         modes = {"auto": 0, "columns": 1, "flow": 2}
         if mode not in modes:
             raise ValueError(f"unknown panel mode {mode!r}")
+        folds = {"auto": 0, "panel": 1, "gemm": 2}
+        if fold is not None and fold not in folds:
+            raise ValueError(f"unknown block-update placement {fold!r}")
         _lib.check(self.lib.bta_ws_set_option(self._ws, 3, modes[mode]), "set_option")
         _lib.check(self.lib.bta_ws_set_option(self._ws, 4, int(helpers)), "set_option")
         if delta is not None:
             _lib.check(self.lib.bta_ws_set_option(self._ws, 5, int(delta)), "set_option")
+        if fold is not None:
+            _lib.check(self.lib.bta_ws_set_option(self._ws, 10, folds[fold]), "set_option")
 
     def set_batch_lanes(self, lanes: int = 0) -> None:
         """Problems factored side by side when this solver is the first member

@@ -20,9 +20,9 @@ def _rel(x, y):
     return float(np.abs(np.asarray(x) - np.asarray(y)).max() / max(np.abs(y).max(), 1e-300))
 
 
-def _factor(kf, mode, helpers=0, delta=None, use_graph=False, calls=1):
+def _factor(kf, mode, helpers=0, delta=None, use_graph=False, calls=1, fold=None):
     s = SerinvSolver(None, kf.b, kf.n, kf.a, use_graph=use_graph)
-    s.set_panel(mode, helpers, delta)
+    s.set_panel(mode, helpers, delta, fold)
     out = []
     for _ in range(calls):
         s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if kf.a else None, kf.Q_T if kf.a else None)
@@ -63,6 +63,35 @@ def test_flow_matches_known_factor_and_columns(cuda, n, b, a, helpers, delta):
     assert _rel(rf["D"], rc["D"]) < 1e-12
     if n > 1:
         assert _rel(rf["L"], rc["L"]) < 1e-12
+
+
+@pytest.mark.parametrize("n,b,a,helpers", [(3, 130, 6, 0), (2, 300, 4, 5), (3, 300, 0, 0), (1, 200, 6, 0),
+                                             (2, 1000, 9, 64), (3, 3200, 6, 0)])
+def test_flow_block_update_as_gemm(cuda, n, b, a, helpers):
+    """The persistent panel without its next-block tasks, the block update
+    then applied by one grouped-GEMM launch per block: the known factor, and
+    the folded form to rounding (the update's sums run in another order).
+    b = 3200 (50 tile columns) is where the automatic choice switches."""
+    print(f"[flow] update as GEMM n={n} b={b} a={a} helpers={helpers}", flush=True)
+    if b <= 2000:
+        kf = known_factor_bta(n, b, a, seed=200 + n + b + a)
+    else:  # off-diagonal draws scaled to the b = 92 block norms (else Q is singular in FP64)
+        from types import SimpleNamespace
+        from pyinla_amd.workloads import known_factor_bta_torch
+        (Ld, Ll, Lb, Lt), (QD, QL, QB, QT) = known_factor_bta_torch(n, b, a, seed=7, device=cuda, norm_b=92)
+        kf = SimpleNamespace(n=n, b=b, a=a, Q_D=QD.cpu().numpy(), Q_L=QL.cpu().numpy(), Q_B=QB.cpu().numpy(),
+                             Q_T=QT.cpu().numpy(), Lf_D=Ld.cpu().numpy(), Lf_L=Ll.cpu().numpy(),
+                             Lf_B=Lb.cpu().numpy(), Lf_T=Lt.cpu().numpy())
+        del Ld, Ll, Lb, Lt, QD, QL, QB, QT
+    rg = _factor(kf, "flow", helpers, fold="gemm")[0]
+    _check_known(kf, rg)
+    rp = _factor(kf, "flow", helpers, fold="panel")[0]
+    assert abs(rg["logdet"] - rp["logdet"]) <= 1e-12 * abs(rp["logdet"])
+    assert _rel(rg["D"], rp["D"]) < 1e-12
+    if n > 1:
+        assert _rel(rg["L"], rp["L"]) < 1e-12
+    ra = _factor(kf, "flow", helpers, fold="auto")[0]
+    assert _rel(ra["D"], rp["D"] if b <= 3072 else rg["D"]) == 0.0
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
