@@ -136,10 +136,20 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
     xcd_tile(local, prefix[jid], tm, tn, tr, tc);
   }
   const int rows = min(64, g.m - tr * 64), cols = min(64, g.n - tc * 64);
-  double* ct = g.C + (int64_t)(tr * 64) * g.ldc + tc * 64;
+  // A tile at a ragged edge of a job at least 64 wide works on the 64-wide
+  // window that ends at the edge (origin r0 / c0, overlapping the previous
+  // tile) and writes only its own rows / columns: every chunk of it then
+  // takes the same unguarded loads as an interior tile (measured: guarded
+  // edge tiles held a 2000 x 2000 product at 397 us against 339 us for
+  // 1984 x 1984).  Overlap rows and columns may miss K terms outside the
+  // tile's own triangular range; they are never stored.
+  const int r0 = (rows < 64 && g.m >= 64) ? g.m - 64 : tr * 64;
+  const int c0 = (cols < 64 && g.n >= 64) ? g.n - 64 : tc * 64;
+  const int wrows = min(64, g.m - r0), wcols = min(64, g.n - c0);
+  double* ct = g.C + (int64_t)r0 * g.ldc + c0;
   TileAcc acc;
   if (g.beta != 0.0) {
-    acc_load(acc, ct, g.ldc, rows, cols);
+    acc_load(acc, ct, g.ldc, wrows, wcols);
     if (g.beta != 1.0) {
 #pragma unroll
       for (int x = 0; x < 2; ++x)
@@ -185,10 +195,10 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
     const int kw = min(BTA_GEMM_KC, kmax - k0);
     const int ma = (sg.tri & TRI_A_LOWER) ? MASK_K_GT_I : ((sg.tri & TRI_A_UPPER) ? MASK_K_LT_I : MASK_NONE);
     const int mb = (sg.tri & TRI_B_LOWER) ? MASK_K_LT_I : ((sg.tri & TRI_B_UPPER) ? MASK_K_GT_I : MASK_NONE);
-    if (sg.ta == 0) pa.load(sg.A + (int64_t)(tr * 64) * sg.lda + k0, sg.lda, 0, rows, kw, ma, tr * 64, k0);
-    else pa.load(sg.A + (int64_t)k0 * sg.lda + tr * 64, sg.lda, 1, rows, kw, ma, tr * 64, k0);
-    if (sg.tb == 1) pb.load(sg.B + (int64_t)(tc * 64) * sg.ldb + k0, sg.ldb, 0, cols, kw, mb, tc * 64, k0);
-    else pb.load(sg.B + (int64_t)k0 * sg.ldb + tc * 64, sg.ldb, 1, cols, kw, mb, tc * 64, k0);
+    if (sg.ta == 0) pa.load(sg.A + (int64_t)r0 * sg.lda + k0, sg.lda, 0, wrows, kw, ma, r0, k0);
+    else pa.load(sg.A + (int64_t)k0 * sg.lda + r0, sg.lda, 1, wrows, kw, ma, r0, k0);
+    if (sg.tb == 1) pb.load(sg.B + (int64_t)c0 * sg.ldb + k0, sg.ldb, 0, wcols, kw, mb, c0, k0);
+    else pb.load(sg.B + (int64_t)k0 * sg.ldb + c0, sg.ldb, 1, wcols, kw, mb, c0, k0);
   };
   bool have = (g.nseg > 0) && next_chunk(true);
   if (have) fetch();
@@ -216,17 +226,18 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = acc_row(x, r), j = acc_col(y);
-        if (i >= rows || j >= cols) continue;
+        const int gi = r0 + i, gj = c0 + j;  // global row / column
+        if (i >= wrows || j >= wcols || gi < tr * 64 || gj < tc * 64) continue;
         const double v = acc.c[x][y][r];
-        gdouble* cg = gview(ct);
+        gdouble* cg = gview(g.C);
         if (g.lower && tr == tc) {
-          if (i >= j) {
-            cg[(int64_t)i * g.ldc + j] = v;
-            if (g.mirror && i > j) cg[(int64_t)j * g.ldc + i] = v;
+          if (gi >= gj) {
+            cg[(int64_t)gi * g.ldc + gj] = v;
+            if (g.mirror && gi > gj) cg[(int64_t)gj * g.ldc + gi] = v;
           }
         } else {
-          cg[(int64_t)i * g.ldc + j] = v;
-          if (g.lower && g.mirror) gview(g.C)[(int64_t)(tc * 64 + j) * g.ldc + tr * 64 + i] = v;
+          cg[(int64_t)gi * g.ldc + gj] = v;
+          if (g.lower && g.mirror) cg[(int64_t)gj * g.ldc + gi] = v;
         }
       }
 }
