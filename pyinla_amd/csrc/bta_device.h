@@ -31,6 +31,12 @@ __device__ __forceinline__ gdouble* gview(double* p) { return (gdouble*)p; }
 __device__ __forceinline__ f64x4 mfma_f64(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// D = -A*B + C through the instruction's negation modifier (neg:[1,0,0]; the
+// blgp field of the F64 MFMA), so no VALU negation sits between an LDS read
+// and its MFMA.
+__device__ __forceinline__ f64x4 mfma_f64_nega(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+}
 
 struct TileAcc {
   f64x4 c[2][2];
@@ -150,12 +156,18 @@ __device__ __forceinline__ void mma_nt(TileAcc& t, const double* __restrict__ sA
   const double* b1p = sB + (wn + 16 + li) * BTA_LDS_LD + lk;
 #pragma unroll 4
   for (int k = kbeg; k < kend; k += 4) {
-    double a0 = a0p[k], a1 = a1p[k], b0 = b0p[k], b1 = b1p[k];
-    if (NEG) { a0 = -a0; a1 = -a1; }
-    t.c[0][0] = mfma_f64(a0, b0, t.c[0][0]);
-    t.c[0][1] = mfma_f64(a0, b1, t.c[0][1]);
-    t.c[1][0] = mfma_f64(a1, b0, t.c[1][0]);
-    t.c[1][1] = mfma_f64(a1, b1, t.c[1][1]);
+    const double a0 = a0p[k], a1 = a1p[k], b0 = b0p[k], b1 = b1p[k];
+    if (NEG) {
+      t.c[0][0] = mfma_f64_nega(a0, b0, t.c[0][0]);
+      t.c[0][1] = mfma_f64_nega(a0, b1, t.c[0][1]);
+      t.c[1][0] = mfma_f64_nega(a1, b0, t.c[1][0]);
+      t.c[1][1] = mfma_f64_nega(a1, b1, t.c[1][1]);
+    } else {
+      t.c[0][0] = mfma_f64(a0, b0, t.c[0][0]);
+      t.c[0][1] = mfma_f64(a0, b1, t.c[0][1]);
+      t.c[1][0] = mfma_f64(a1, b0, t.c[1][0]);
+      t.c[1][1] = mfma_f64(a1, b1, t.c[1][1]);
+    }
   }
 }
 
@@ -169,8 +181,12 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// mma_nt over the full K = 64 of two LDS tiles, unrolled: the operand reads of
-// later k steps are issued while earlier MFMAs run.
+// mma_nt over the full K = 64 of two LDS tiles, unrolled, with the operand
+// fragments of k step t+1 read while the MFMAs of step t run (two fragment
+// register sets; the scheduling barriers keep each step's reads ahead of the
+// previous step's MFMAs, which LLVM otherwise sinks behind them and then
+// waits for).  A helper runs one wave per SIMD, so nothing else hides the
+// LDS latency.
 template <bool NEG>
 __device__ __forceinline__ void mma_nt64(TileAcc& t, const double* __restrict__ sA,
                                          const double* __restrict__ sB) {
@@ -181,14 +197,35 @@ __device__ __forceinline__ void mma_nt64(TileAcc& t, const double* __restrict__ 
   const double* a1p = sA + (wm + 16 + li) * BTA_LDS_LD + lk;
   const double* b0p = sB + (wn + li) * BTA_LDS_LD + lk;
   const double* b1p = sB + (wn + 16 + li) * BTA_LDS_LD + lk;
+  double a0 = a0p[0], a1 = a1p[0], b0 = b0p[0], b1 = b1p[0];
 #pragma unroll
   for (int k = 0; k < 64; k += 4) {
-    double a0 = a0p[k], a1 = a1p[k], b0 = b0p[k], b1 = b1p[k];
-    if (NEG) { a0 = -a0; a1 = -a1; }
-    t.c[0][0] = mfma_f64(a0, b0, t.c[0][0]);
-    t.c[0][1] = mfma_f64(a0, b1, t.c[0][1]);
-    t.c[1][0] = mfma_f64(a1, b0, t.c[1][0]);
-    t.c[1][1] = mfma_f64(a1, b1, t.c[1][1]);
+    double n0 = 0.0, n1 = 0.0, m0 = 0.0, m1 = 0.0;
+    if (k + 4 < 64) {
+      n0 = a0p[k + 4];
+      n1 = a1p[k + 4];
+      m0 = b0p[k + 4];
+      m1 = b1p[k + 4];
+    }
+    if (NEG) {
+      t.c[0][0] = mfma_f64_nega(a0, b0, t.c[0][0]);
+      t.c[0][1] = mfma_f64_nega(a0, b1, t.c[0][1]);
+      t.c[1][0] = mfma_f64_nega(a1, b0, t.c[1][0]);
+      t.c[1][1] = mfma_f64_nega(a1, b1, t.c[1][1]);
+    } else {
+      t.c[0][0] = mfma_f64(a0, b0, t.c[0][0]);
+      t.c[0][1] = mfma_f64(a0, b1, t.c[0][1]);
+      t.c[1][0] = mfma_f64(a1, b0, t.c[1][0]);
+      t.c[1][1] = mfma_f64(a1, b1, t.c[1][1]);
+    }
+    if (k + 4 < 64) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // the 4 LDS reads of step k+4
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);  // then the 4 MFMAs of step k
+    }
+    a0 = n0;
+    a1 = n1;
+    b0 = m0;
+    b1 = m1;
   }
 }
 

@@ -568,7 +568,7 @@ __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA
       acc_to_lds(acc, sA);
     }
     lds_barrier();
-    mma_nt<true>(accd, sA, sA, 0, 64);
+    mma_nt64<true>(accd, sA, sA);
     flow_publish(xflag(p, k + 1, k), tag);
     acc_to_lds(accd, sA);
     __syncthreads();

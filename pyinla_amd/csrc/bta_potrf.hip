@@ -133,7 +133,7 @@ __device__ __forceinline__ void panel_body(const PanelArgs& p, int idx, double* 
     lds_load_tile(sA, pa, b, rows, 64);
     lds_load_tile(sB, pb, b, cols, 64);
     __syncthreads();
-    mma_nt<true>(acc, sA, sB, 0, 64);
+    mma_nt64<true>(acc, sA, sB);
     __syncthreads();
   }
 
