@@ -16,12 +16,11 @@ namespace bta {
 #define BTA_GEMM_MINB 4  // resident workgroups per CU the register budget is cut for
 #endif
 
-// acc += alpha * sA(64 x K) * sB(64 x K)^T over k in [0, kend) (LDS stride LD).
-// KFIX > 0: kend == KFIX is known at compile time (fully unrolled).
-template <int LD = BTA_LDS_LD, int KFIX = 0>
-__device__ __forceinline__ void mma_nt_alpha(TileAcc& t, const double* __restrict__ sA,
-                                             const double* __restrict__ sB, int kend, double alpha) {
-  if (KFIX > 0) kend = KFIX;
+// acc += sA(64 x K) * sB(64 x K)^T over k in [0, kend) (LDS stride LD; any
+// alpha was applied to sA when it was stored).
+template <int LD>
+__device__ __forceinline__ void mma_nt_ld(TileAcc& t, const double* __restrict__ sA,
+                                          const double* __restrict__ sB, int kend) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
@@ -31,7 +30,7 @@ __device__ __forceinline__ void mma_nt_alpha(TileAcc& t, const double* __restric
   const double* b1p = sB + (wn + 16 + li) * LD + lk;
 #pragma unroll 4
   for (int k = 0; k < kend; k += 4) {
-    const double a0 = alpha * a0p[k], a1 = alpha * a1p[k], b0 = b0p[k], b1 = b1p[k];
+    const double a0 = a0p[k], a1 = a1p[k], b0 = b0p[k], b1 = b1p[k];
     t.c[0][0] = mfma_f64(a0, b0, t.c[0][0]);
     t.c[0][1] = mfma_f64(a0, b1, t.c[0][1]);
     t.c[1][0] = mfma_f64(a1, b0, t.c[1][0]);
@@ -112,12 +111,13 @@ struct ChunkPrefetch {
       }
     }
   }
-  __device__ __forceinline__ void store(double* __restrict__ s, int trans) const {
+  __device__ __forceinline__ void store(double* __restrict__ s, int trans, double alpha = 1.0) const {
 #pragma unroll
     for (int q = 0; q < N; ++q) {
       const int e = threadIdx.x + BTA_THREADS * q;
-      if (trans == 0) s[(e / BTA_GEMM_KC) * BTA_GEMM_LD + (e % BTA_GEMM_KC)] = r[q];
-      else s[(e & 63) * BTA_GEMM_LD + (e >> 6)] = r[q];
+      const double v = alpha * r[q];
+      if (trans == 0) s[(e / BTA_GEMM_KC) * BTA_GEMM_LD + (e % BTA_GEMM_KC)] = v;
+      else s[(e & 63) * BTA_GEMM_LD + (e >> 6)] = v;
     }
   }
 };
@@ -205,17 +205,12 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
   while (have) {
     const GemmSeg& sg = g.seg[seg_id];
     const int kw = min(BTA_GEMM_KC, kmax - k0);
-    const double alpha = sg.alpha;
-    pa.store(sA, sg.ta);
+    pa.store(sA, sg.ta, sg.alpha);
     pb.store(sB, sg.tb == 1 ? 0 : 1);
     __syncthreads();
     have = next_chunk(false);
     if (have) fetch();
-#ifdef BTA_GEMM_UNROLL
-    if (kw == BTA_GEMM_KC) mma_nt_alpha<BTA_GEMM_LD, BTA_GEMM_KC>(acc, sA, sB, BTA_GEMM_KC, alpha);
-    else
-#endif
-      mma_nt_alpha<BTA_GEMM_LD>(acc, sA, sB, (kw + 3) & ~3, alpha);
+    mma_nt_ld<BTA_GEMM_LD>(acc, sA, sB, (kw + 3) & ~3);
     lds_barrier();  // not a wait for the next chunk's loads: its stores wait for them
   }
   // epilogue
