@@ -662,25 +662,9 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
       fetched = false;
       continue;
     }
-    if (!fetched) {
-      if (!flow_wait(p, cur.f1, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
-      if (cur.f2 && !flow_wait(p, cur.f2, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
-      step_fetch(cur, pa, pb);
-    }
-    // the following step.  Thread 0 issues the loads of its flags now, both
-    // at once, so that their latency overlaps the staging of this step.
-    HStep nxt;
-    bool has_next = true;
-    if (cur.op == FLOW_NEXT && cur.kk + 1 < flow_next_end(cur.k, nT)) nxt = make_step(p, cur.t, cur.kk + 1);
-    else if (cur.t + 1 < t1) nxt = make_step(p, cur.t + 1, -1);
-    else has_next = false;
-    const bool probe = has_next && nxt.op != FLOW_ZERO;
-    unsigned long long fv1 = 0, fv2 = 0;
-    if (threadIdx.x == 0 && probe) {
-      fv1 = flag_value(nxt.f1, tag);
-      fv2 = flag_value(nxt.f2, tag);
-    }
-    // accumulator of the step's target
+    // accumulator of the step's target: its data come from this helper's own
+    // earlier steps, so its loads are issued before the wait for the operands
+    // and their latency overlaps it
     double* tg = g;
     int64_t ld = p.b;
     int trows = rows, tcols = cols;
@@ -706,6 +690,24 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
         if (hand && cur.k > 0) acc_load_sys(acc, cur.rho == cur.c ? hdiag(p, cur.c) : hsub(p, cur.c));
         else acc_load(acc, g, p.b, rows, cols);
       }
+    }
+    if (!fetched) {
+      if (!flow_wait(p, cur.f1, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
+      if (cur.f2 && !flow_wait(p, cur.f2, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
+      step_fetch(cur, pa, pb);
+    }
+    // the following step.  Thread 0 issues the loads of its flags now, both
+    // at once, so that their latency overlaps the staging of this step.
+    HStep nxt;
+    bool has_next = true;
+    if (cur.op == FLOW_NEXT && cur.kk + 1 < flow_next_end(cur.k, nT)) nxt = make_step(p, cur.t, cur.kk + 1);
+    else if (cur.t + 1 < t1) nxt = make_step(p, cur.t + 1, -1);
+    else has_next = false;
+    const bool probe = has_next && nxt.op != FLOW_ZERO;
+    unsigned long long fv1 = 0, fv2 = 0;
+    if (threadIdx.x == 0 && probe) {
+      fv1 = flag_value(nxt.f1, tag);
+      fv2 = flag_value(nxt.f2, tag);
     }
     // stage operands in LDS
     if (cur.op == FLOW_TRSM) acc_to_lds(acc, sA);
