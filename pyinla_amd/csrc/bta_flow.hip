@@ -440,30 +440,39 @@ __device__ __forceinline__ double* hdiag(const FlowArgs& p, int c) {
   return p.hacc + ((int64_t)p.nT + c) * 4096;
 }
 
-// Bounded wait for *flag == tag.  Gives up (returns false and raises err) after
-// spin_ticks, or at once when another workgroup has raised err, so that every
-// workgroup still reaches the end of its list.
-__device__ __forceinline__ bool flow_wait(const FlowArgs& p, unsigned long long* flag, unsigned long long tag,
-                                          int* ok_sh) {
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != tag) {
-      const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
-      while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != tag) {
-        if (__hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) { ok = 0; break; }
-        if (__builtin_amdgcn_s_memrealtime() - t0 > p.spin_ticks) {
-          atomicOr(p.err, 1);
-          ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
+// Bounded wait by thread 0 for *flag == tag (nullptr: nothing to wait for).
+// Gives up (returns 0 and raises err) after spin_ticks, or at once when another
+// workgroup has raised err, so that every workgroup still reaches the end of
+// its list.
+__device__ __forceinline__ int flow_poll(const FlowArgs& p, unsigned long long* flag, unsigned long long tag) {
+  if (flag == nullptr) return 1;
+  if (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == tag) return 1;
+  const unsigned long long t0 = __builtin_amdgcn_s_memrealtime();
+  while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != tag) {
+    if (__hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) return 0;
+    if (__builtin_amdgcn_s_memrealtime() - t0 > p.spin_ticks) {
+      atomicOr(p.err, 1);
+      return 0;
     }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return 1;
+}
+
+// The workgroup waits until both flags are set (f2 may be nullptr).  The
+// barriers order LDS only: the guarded data are read after the second one in
+// program order, and a full barrier would also wait for every global load the
+// workgroup has in flight (the prefetched accumulator of the step).
+__device__ __forceinline__ bool flow_wait2(const FlowArgs& p, unsigned long long* f1, unsigned long long* f2,
+                                           unsigned long long tag, int* ok_sh) {
+  if (threadIdx.x == 0) {
+    int ok = flow_poll(p, f1, tag);
+    if (ok) ok = flow_poll(p, f2, tag);
     *ok_sh = ok;
   }
-  __syncthreads();
+  lds_barrier();
   const int ok = *ok_sh;
-  __syncthreads();
+  lds_barrier();
   return ok != 0;
 }
 
@@ -552,8 +561,7 @@ __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA
       acc_load(acc, flow_tile(p, 1, 0, rows, cols), p.b, rows, cols);
       acc_load(accd, flow_tile(p, 1, 1, rows, cols), p.b, rows, cols);
     } else {
-      if (!flow_wait(p, hflag(p, k, 0), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 2);
-      if (!flow_wait(p, hflag(p, k, 1), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 2);
+      if (!flow_wait2(p, hflag(p, k, 0), hflag(p, k, 1), tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 2);
       acc_load_sys(acc, hsub(p, k));
       acc_load_sys(accd, hdiag(p, k + 1));
     }
@@ -692,8 +700,7 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
       }
     }
     if (!fetched) {
-      if (!flow_wait(p, cur.f1, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
-      if (cur.f2 && !flow_wait(p, cur.f2, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
+      if (!flow_wait2(p, cur.f1, cur.f2, tag, ok_sh) && threadIdx.x == 0) atomicOr(p.err, 4);
       step_fetch(cur, pa, pb);
     }
     // the following step.  Thread 0 issues the loads of its flags now, both
