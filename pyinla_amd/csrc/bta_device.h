@@ -229,6 +229,52 @@ __device__ __forceinline__ void mma_nt64(TileAcc& t, const double* __restrict__ 
   }
 }
 
+// mma_nt64<true> that also loads the next step's two operand tiles (64 x 64,
+// contiguous) into the prefetch registers, two loads per k step, so that the
+// load issue overlaps the MFMAs instead of preceding them.  Same registers
+// and layout as TilePrefetch::load of a full tile.
+__device__ __forceinline__ void mma_nt64_neg_pf(TileAcc& t, const double* __restrict__ sA,
+                                                const double* __restrict__ sB, double (&ra)[16],
+                                                const double* __restrict__ srcA, double (&rb)[16],
+                                                const double* __restrict__ srcB) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const double* a0p = sA + (wm + li) * BTA_LDS_LD + lk;
+  const double* a1p = sA + (wm + 16 + li) * BTA_LDS_LD + lk;
+  const double* b0p = sB + (wn + li) * BTA_LDS_LD + lk;
+  const double* b1p = sB + (wn + 16 + li) * BTA_LDS_LD + lk;
+  const gdouble* ga = gview(srcA) + (threadIdx.x >> 6) * 64 + lane;
+  const gdouble* gb = gview(srcB) + (threadIdx.x >> 6) * 64 + lane;
+  double a0 = a0p[0], a1 = a1p[0], b0 = b0p[0], b1 = b1p[0];
+#pragma unroll
+  for (int k = 0; k < 64; k += 4) {
+    double n0 = 0.0, n1 = 0.0, m0 = 0.0, m1 = 0.0;
+    if (k + 4 < 64) {
+      n0 = a0p[k + 4];
+      n1 = a1p[k + 4];
+      m0 = b0p[k + 4];
+      m1 = b1p[k + 4];
+    }
+    const int j = k >> 2;  // prefetch row group: rows 4 j + (thread / 64)
+    ra[j] = ga[j * 256];
+    rb[j] = gb[j * 256];
+    t.c[0][0] = mfma_f64_nega(a0, b0, t.c[0][0]);
+    t.c[0][1] = mfma_f64_nega(a0, b1, t.c[0][1]);
+    t.c[1][0] = mfma_f64_nega(a1, b0, t.c[1][0]);
+    t.c[1][1] = mfma_f64_nega(a1, b1, t.c[1][1]);
+    if (k + 4 < 64) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // LDS reads of step k+4
+      __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);  // two prefetch loads
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);  // MFMAs of step k
+    }
+    a0 = n0;
+    a1 = n1;
+    b0 = m0;
+    b1 = m1;
+  }
+}
+
 // Load / store the accumulator from / to a row-major global block (masked).
 // Addresses are formed as one per-lane base pointer plus wave-uniform offsets
 // (16x + 4r) * ld + 16y, so that only the base occupies vector registers.

@@ -724,7 +724,9 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
     if (threadIdx.x == 0) *ok_sh = (probe && fv1 == tag && fv2 == tag) ? 1 : 0;
     lds_barrier();
     fetched = (*ok_sh != 0);
-    if (fetched) step_fetch(nxt, pa, pb);
+    // an update followed by an update: its operand loads ride in the MFMA loop
+    const bool pf_mma = fetched && cur.op != FLOW_TRSM && nxt.srcA != nullptr;
+    if (fetched && !pf_mma) step_fetch(nxt, pa, pb);
     const double* sBop = (cur.srcB == cur.srcA) ? sA : sB;
     if (cur.op == FLOW_TRSM) {
       flow_trsm(acc, sA, sB);  // the accumulator was staged in sA: reuse its registers
@@ -734,7 +736,8 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
       carry_rho = carry_c = -1;
       flow_publish(xflag(p, cur.rho, cur.c), tag);
     } else {
-      mma_nt64<true>(acc, sA, sBop);
+      if (pf_mma) mma_nt64_neg_pf(acc, sA, sBop, pa.r, nxt.srcA, pb.r, nxt.srcB);
+      else mma_nt64<true>(acc, sA, sBop);
       lds_barrier();
       if (cur.op == FLOW_NEXT) {
         if (!(has_next && nxt.op == FLOW_NEXT && nxt.t == cur.t)) acc_store(acc, tg, ld, trows, tcols);
