@@ -516,16 +516,28 @@ __device__ __forceinline__ void flow_trsm(TileAcc& x, const double* sC, const do
 // Chain: store the factored diagonal tile k held in sA (L_kk) / sB (inverse).
 // The flag is published separately (chain_flag), so that the write-through
 // stores drain while the chain already computes.
-__device__ __forceinline__ void chain_store_diag(const FlowArgs& p, int k, const TileCholResult& res,
-                                                 const double* sA, const double* sB) {
+__device__ __forceinline__ void chain_store_inv(const FlowArgs& p, int k, const double* sB) {
+  double* w = p.winv + (int64_t)k * 4096;
+  const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+#pragma unroll
+  for (int h = 0; h < 16; h += 8) {  // two halves of 8 rows: reads first, then stores
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = sB[(r0 + 4 * (h + j)) * BTA_LDS_LD + c];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) st_sys(&w[(r0 + 4 * (h + j)) * 64 + c], v[j]);
+  }
+}
+
+// ... then, after the inverse's flag, L_kk to the factor (read by no helper)
+// and the tile's log pivots.
+__device__ __forceinline__ void chain_store_l(const FlowArgs& p, int k, const TileCholResult& res,
+                                              const double* sA) {
   int rows, cols;
   double* g = flow_tile(p, k, k, rows, cols);
-  double* w = p.winv + (int64_t)k * 4096;
-  const int c = threadIdx.x & 63;
-  for (int r = threadIdx.x >> 6; r < 64; r += 4) {
+  const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+  for (int r = r0; r < 64; r += 4)
     if (r < rows && c < cols) g[(int64_t)r * p.b + c] = sA[r * BTA_LDS_LD + c];
-    st_sys(&w[r * 64 + c], sB[r * BTA_LDS_LD + c]);
-  }
   if (threadIdx.x == 0) {
     p.logpart[k] = res.logsum;
     if (res.fail >= 0) atomicMin(p.info, (unsigned long long)(p.row0 + (long long)k * 64 + res.fail));
@@ -538,10 +550,11 @@ __device__ __forceinline__ void chain_store_diag(const FlowArgs& p, int k, const
 //   2. solves X = C(k+1, k) inv(L_kk)^T and issues the stores of X;
 //   3. applies C(k+1, k+1) -= X X^T, then publishes X (its stores drained
 //      during the update);
-//   4. factors C(k+1, k+1), stores L and its inverse and publishes the
-//      inverse at once: the helpers' solves of column k+1 wait for it, and the
-//      stores drain while they are issued (a few hundred cycles of wait,
-//      measured on MI355X).
+//   4. factors C(k+1, k+1), stores its inverse and publishes it at once (the
+//      helpers' solves of column k+1 wait for it; the stores drain while
+//      they are issued, a few hundred cycles of wait measured on MI355X),
+//      then stores L into the factor (no helper reads it; those stores drain
+//      during the next step's wait).
 __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA, double* sB,
                            int* ok_sh) {
   const int nT = p.nT;
@@ -553,8 +566,9 @@ __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA
     acc_to_lds(acc, sA);
     __syncthreads();
     const TileCholResult res = tile_chol_inv(sA, sB, rows);
-    chain_store_diag(p, 0, res, sA, sB);
+    chain_store_inv(p, 0, sB);
     flow_publish(wflag(p, 0), tag);
+    chain_store_l(p, 0, res, sA);
   }
   for (int k = 0; k + 1 < nT; ++k) {
     if (k == 0) {
@@ -582,8 +596,9 @@ __device__ void flow_chain(const FlowArgs& p, unsigned long long tag, double* sA
     __syncthreads();
     const int mv = min(64, p.b - 64 * (k + 1));
     const TileCholResult res = tile_chol_inv(sA, sB, mv);
-    chain_store_diag(p, k + 1, res, sA, sB);
+    chain_store_inv(p, k + 1, sB);
     flow_publish(wflag(p, k + 1), tag);
+    chain_store_l(p, k + 1, res, sA);
   }
 }
 
