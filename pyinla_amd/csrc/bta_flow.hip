@@ -154,16 +154,16 @@ static void flow_build_levels(const FlowShape& s, int helpers, int delta, FlowSc
 #define FLOW_COST_CTRSM 10000  // chain: solve of the sub-diagonal tile, stores
 #endif
 #ifndef FLOW_COST_CUPD
-#define FLOW_COST_CUPD 5300  // chain: update of the next diagonal tile
+#define FLOW_COST_CUPD 4600  // chain: update of the next diagonal tile
 #endif
 #ifndef FLOW_COST_UPD
-#define FLOW_COST_UPD 12600  // rank-64 update, accumulator loaded and stored
+#define FLOW_COST_UPD 12000  // rank-64 update, accumulator loaded and stored
 #endif
 #ifndef FLOW_COST_CARRY
 #define FLOW_COST_CARRY 3000  // saved when the previous task left the accumulator in registers
 #endif
 #ifndef FLOW_COST_TRSM
-#define FLOW_COST_TRSM 12500
+#define FLOW_COST_TRSM 14500  // tile solve, exchange copy and flag
 #endif
 #ifndef FLOW_COST_NEXTCOL
 #define FLOW_COST_NEXTCOL 10800  // one column of a next-block task
