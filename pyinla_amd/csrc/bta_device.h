@@ -14,6 +14,7 @@
 #define BTA_LDS_LD 66  // padded LDS row stride in doubles (bank-conflict-free fragment reads)
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 // Global-address-space views of pointers.  Pointers read from memory (job
 // tables) are generic to the compiler, which then emits FLAT accesses; a FLAT
@@ -23,6 +24,7 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) double gdouble;
 __device__ __forceinline__ const gdouble* gview(const double* p) { return (const gdouble*)p; }
 __device__ __forceinline__ gdouble* gview(double* p) { return (gdouble*)p; }
+typedef __attribute__((address_space(1))) f64x2 gf64x2;
 
 // Operand layout of v_mfma_f64_16x16x4f64 (D = A*B + C, A 16x4, B 4x16):
 //   A[i][k] is supplied by lane i + 16*k, B[k][j] by lane j + 16*k,
@@ -140,6 +142,27 @@ struct TilePrefetch {
       for (int j = 0; j < 16; ++j) s[c * BTA_LDS_LD + r0 + 4 * j] = r[j];
     }
   }
+  // A full, contiguous 64 x 64 tile (ld 64) as 128-bit pairs: pair q of the
+  // thread is columns 2 (t % 32), +1 of row t / 32 + 8 q (one wave: two whole
+  // rows per load), half the load and LDS-store instructions of load/store.
+  __device__ __forceinline__ static const gf64x2* full_pair(const double* __restrict__ g, int q) {
+    const int t = threadIdx.x;
+    return (const gf64x2*)(gview(g) + ((t >> 5) + 8 * q) * 64 + 2 * (t & 31));
+  }
+  __device__ __forceinline__ void set_pair(int q, f64x2 v) {
+    r[2 * q] = v.x;
+    r[2 * q + 1] = v.y;
+  }
+  __device__ __forceinline__ void load_full(const double* __restrict__ g) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) set_pair(q, *full_pair(g, q));
+  }
+  __device__ __forceinline__ void store_full(double* __restrict__ s) const {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      *(f64x2*)&s[((t >> 5) + 8 * q) * BTA_LDS_LD + 2 * (t & 31)] = f64x2{r[2 * q], r[2 * q + 1]};
+  }
 };
 
 // acc += sign * sA(64 x K) * sB(64 x K)^T over k in [0, kmax), kmax % 4 == 0.
@@ -230,9 +253,9 @@ __device__ __forceinline__ void mma_nt64(TileAcc& t, const double* __restrict__ 
 }
 
 // mma_nt64<true> that also loads the next step's two operand tiles (64 x 64,
-// contiguous) into the prefetch registers, two loads per k step, so that the
-// load issue overlaps the MFMAs instead of preceding them.  Same registers
-// and layout as TilePrefetch::load of a full tile.
+// contiguous) into the prefetch registers, one 128-bit load per k step, so
+// that the load issue overlaps the MFMAs instead of preceding them.  Same
+// registers and layout as TilePrefetch::load_full.
 __device__ __forceinline__ void mma_nt64_neg_pf(TileAcc& t, const double* __restrict__ sA,
                                                 const double* __restrict__ sB, double (&ra)[16],
                                                 const double* __restrict__ srcA, double (&rb)[16],
@@ -244,8 +267,6 @@ __device__ __forceinline__ void mma_nt64_neg_pf(TileAcc& t, const double* __rest
   const double* a1p = sA + (wm + 16 + li) * BTA_LDS_LD + lk;
   const double* b0p = sB + (wn + li) * BTA_LDS_LD + lk;
   const double* b1p = sB + (wn + 16 + li) * BTA_LDS_LD + lk;
-  const gdouble* ga = gview(srcA) + (threadIdx.x >> 6) * 64 + lane;
-  const gdouble* gb = gview(srcB) + (threadIdx.x >> 6) * 64 + lane;
   double a0 = a0p[0], a1 = a1p[0], b0 = b0p[0], b1 = b1p[0];
 #pragma unroll
   for (int k = 0; k < 64; k += 4) {
@@ -256,16 +277,22 @@ __device__ __forceinline__ void mma_nt64_neg_pf(TileAcc& t, const double* __rest
       m0 = b0p[k + 4];
       m1 = b1p[k + 4];
     }
-    const int j = k >> 2;  // prefetch row group: rows 4 j + (thread / 64)
-    ra[j] = ga[j * 256];
-    rb[j] = gb[j * 256];
+    const int j = k >> 2;  // k step: pair j / 2 of operand A (even) or B (odd)
+    const f64x2 v = *TilePrefetch::full_pair((j & 1) ? srcB : srcA, j >> 1);
+    if (j & 1) {
+      rb[j - 1] = v.x;
+      rb[j] = v.y;
+    } else {
+      ra[j] = v.x;
+      ra[j + 1] = v.y;
+    }
     t.c[0][0] = mfma_f64_nega(a0, b0, t.c[0][0]);
     t.c[0][1] = mfma_f64_nega(a0, b1, t.c[0][1]);
     t.c[1][0] = mfma_f64_nega(a1, b0, t.c[1][0]);
     t.c[1][1] = mfma_f64_nega(a1, b1, t.c[1][1]);
     if (k + 4 < 64) {
       __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // LDS reads of step k+4
-      __builtin_amdgcn_sched_group_barrier(0x020, 2, 0);  // two prefetch loads
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one prefetch load
       __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);  // MFMAs of step k
     }
     a0 = n0;

@@ -658,8 +658,8 @@ __device__ __forceinline__ unsigned long long flag_value(const unsigned long lon
 // cached loads after the flag see them; the L2 lines cannot be older than the
 // launch.
 __device__ __forceinline__ void step_fetch(const HStep& st, TilePrefetch& pa, TilePrefetch& pb) {
-  if (st.srcA) pa.load(st.srcA, 64, 0, 64, 64);
-  if (st.srcB && st.srcB != st.srcA) pb.load(st.srcB, 64, 0, 64, 64);
+  if (st.srcA) pa.load_full(st.srcA);
+  if (st.srcB && st.srcB != st.srcA) pb.load_full(st.srcB);
 }
 
 __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, double* sA, double* sB,
@@ -733,8 +733,8 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
     }
     // stage operands in LDS
     if (cur.op == FLOW_TRSM) acc_to_lds(acc, sA);
-    else pa.store(sA, 0);
-    if (cur.srcB != cur.srcA) pb.store(sB, 0);
+    else pa.store_full(sA);
+    if (cur.srcB != cur.srcA) pb.store_full(sB);
     // the following step's operands are fetched now if they are ready
     if (threadIdx.x == 0) *ok_sh = (probe && fv1 == tag && fv2 == tag) ? 1 : 0;
     lds_barrier();

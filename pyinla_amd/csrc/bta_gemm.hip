@@ -265,8 +265,6 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
 #define G2_LDA (G2_KC + 1)
 #define G2_LDB (64 + 4)
 
-typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(1))) f64x2 gf64x2;
 
 // Four elements of one 64 x 16 operand chunk per thread.  form 0 ("K
 // contiguous"): element (r, k) at g[r * ld + k], the thread owns rows t/8 and
