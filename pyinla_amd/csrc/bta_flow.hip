@@ -739,6 +739,14 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
     if (threadIdx.x == 0) *ok_sh = (probe && fv1 == tag && fv2 == tag) ? 1 : 0;
     lds_barrier();
     fetched = (*ok_sh != 0);
+    // not ready yet: thread 0 loads the flags again now and looks at them
+    // after the MFMAs (a second chance to fetch ahead of the next step)
+    const bool second = probe && !fetched;
+    unsigned long long gv1 = 0, gv2 = 0;
+    if (second && threadIdx.x == 0) {
+      gv1 = flag_value(nxt.f1, tag);
+      gv2 = flag_value(nxt.f2, tag);
+    }
     // an update followed by an update: its operand loads ride in the MFMA loop
     const bool pf_mma = fetched && cur.op != FLOW_TRSM && nxt.srcA != nullptr;
     if (fetched && !pf_mma) step_fetch(nxt, pa, pb);
@@ -772,6 +780,12 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
         carry_rho = carry_c = -1;
       }
     }
+    if (second) {  // ok_sh[1] is written once per step, after a barrier that follows its last read
+      if (threadIdx.x == 0) ok_sh[1] = (gv1 == tag && gv2 == tag) ? 1 : 0;
+      lds_barrier();
+      fetched = (ok_sh[1] != 0);
+      if (fetched) step_fetch(nxt, pa, pb);
+    }
     if (!has_next) break;
     cur = nxt;
   }
@@ -780,10 +794,10 @@ __device__ void flow_helper(const FlowArgs& p, int h, unsigned long long tag, do
 __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow(FlowArgs p) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ int ok_sh;
+  __shared__ int ok_sh[2];
   const unsigned long long tag = (p.epoch[0] << 20) | (unsigned long long)p.blk;
-  if (blockIdx.x == 0) flow_chain(p, tag, sA, sB, &ok_sh);
-  else flow_helper(p, blockIdx.x - 1, tag, sA, sB, &ok_sh);
+  if (blockIdx.x == 0) flow_chain(p, tag, sA, sB, ok_sh);
+  else flow_helper(p, blockIdx.x - 1, tag, sA, sB, ok_sh);
 }
 
 // -------------------------------------------------------------------- host
@@ -923,7 +937,7 @@ void flow_launch_block(const Workspace& ws, const FlowPlan& fp, const FlowSched&
 __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow_batched(FlowBatch fb) {
   __shared__ double sA[64 * BTA_LDS_LD];
   __shared__ double sB[64 * BTA_LDS_LD];
-  __shared__ int ok_sh;
+  __shared__ int ok_sh[2];
   const int wg = (int)blockIdx.x;
   int pos = wg;  // position in the entries' ranges of `begin`
   if (fb.iso) {
@@ -945,8 +959,8 @@ __global__ void __launch_bounds__(BTA_THREADS, 2) k_panel_flow_batched(FlowBatch
   }
   const FlowArgs p = fb.e[lo];
   const unsigned long long tag = (p.epoch[0] << 20) | (unsigned long long)p.blk;
-  if (local == 0) flow_chain(p, tag, sA, sB, &ok_sh);
-  else flow_helper(p, local - 1, tag, sA, sB, &ok_sh);
+  if (local == 0) flow_chain(p, tag, sA, sB, ok_sh);
+  else flow_helper(p, local - 1, tag, sA, sB, ok_sh);
 }
 
 void flow_launch_batched(const FlowBatch& fb, int total, hipStream_t st) {
