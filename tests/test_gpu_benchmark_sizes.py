@@ -47,7 +47,7 @@ def _rel(x, y):
 
 
 def _log(msg):
-    print(f"[baseline-sizes] {msg}", flush=True)
+    print(f"[benchmark-sizes] {msg}", flush=True)
 
 
 def _check_f_parts(parts, ref):
