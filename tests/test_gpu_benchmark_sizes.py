@@ -73,6 +73,18 @@ def test_st_model_at_benchmark_block_size(cuda, nx, ny):
     assert abs(f_adj - ref["f"]) <= 1e-10 * max(abs(ref["ld_c"]), abs(ref["f"]))
     _, g_fd = om.fd_gradient(prob, cm.table, prob.theta, eps=1e-4)
     assert _rel(g_adj, g_fd) < 1e-6, (g_adj, g_fd)
+    # marginal variances per entry: diag(Sigma) of Q_cond through the explicit
+    # block inverses at this block size (the oracle's agrees with a dense
+    # inverse to ~2e-13 per entry on the 40 x 40 mesh, kappa(Q_cond) ~ 8e3)
+    _log("marginal variances vs oracle, per entry")
+    lat, obs = model.marginal_variances(prob.theta)
+    lat_ref, obs_ref = om.marginal_variances(prob, cm.table, prob.theta)
+    lat = lat.cpu().numpy()
+    assert np.all(lat_ref > 0)
+    assert np.all(np.abs(lat - lat_ref) <= 1e-9 * lat_ref), np.max(np.abs(lat - lat_ref) / lat_ref)
+    for v, r in zip(obs, obs_ref):
+        v = v.cpu().numpy()
+        assert np.all(np.abs(v - r) <= 1e-9 * np.abs(r) + 1e-12 * np.abs(r).max())
 
 
 @pytest.mark.timeout(900)

@@ -60,12 +60,18 @@ def test_selected_inversion_matches_oracle(cuda, n, b, a):
     B = kf.Lf_B.copy() if a else None
     T = kf.Lf_T.copy() if a else None
     ob.pobtasi(D, L, B, T)
-    assert _rel(s.A_diagonal_blocks.cpu().numpy(), D) < RTOL
+    Dg = s.A_diagonal_blocks.cpu().numpy()
+    assert _rel(Dg, D) < RTOL
+    # the variances (diagonal of Sigma) per entry as well
+    dvar, dvar_g = np.diagonal(D, axis1=1, axis2=2), np.diagonal(Dg, axis1=1, axis2=2)
+    assert np.all(dvar > 0) and np.all(np.abs(dvar_g - dvar) <= RTOL * dvar)
     if n > 1:
         assert _rel(s.A_lower_diagonal_blocks.cpu().numpy(), L) < RTOL
     if a:
         assert _rel(s.A_arrow_bottom_blocks.cpu().numpy(), B) < RTOL
         assert _rel(s.A_arrow_tip_block.cpu().numpy(), T) < RTOL
+        tvar = np.diag(T)
+        assert np.all(np.abs(np.diag(s.A_arrow_tip_block.cpu().numpy()) - tvar) <= RTOL * tvar)
 
 
 def test_bt_solve_and_selinv_on_bta_buffers(cuda):
