@@ -382,6 +382,26 @@ __device__ __forceinline__ double ld_sys(const double* p) {
   return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// A 64 x 64 accumulator tile to / from a contiguous 64 x 64 buffer with
+// system-scope stores / loads.
+__device__ __forceinline__ void acc_load_sys(TileAcc& t, const double* __restrict__ g) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) t.c[x][y][r] = ld_sys(&g[acc_row(x, r) * 64 + acc_col(y)]);
+}
+
+__device__ __forceinline__ void acc_store_sys(const TileAcc& t, double* __restrict__ g) {
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) st_sys(&g[acc_row(x, r) * 64 + acc_col(y)], t.c[x][y][r]);
+}
+
 // All threads call after their st_sys stores of the guarded data.
 __device__ __forceinline__ void flag_publish(unsigned long long* flag, unsigned long long epoch) {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);

@@ -477,24 +477,6 @@ __device__ __forceinline__ bool flow_wait2(const FlowArgs& p, unsigned long long
 }
 
 // 64x64 contiguous tile <-> registers / LDS with system-scope accesses.
-__device__ __forceinline__ void acc_load_sys(TileAcc& t, const double* __restrict__ g) {
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) t.c[x][y][r] = ld_sys(&g[acc_row(x, r) * 64 + acc_col(y)]);
-}
-
-__device__ __forceinline__ void acc_store_sys(const TileAcc& t, double* __restrict__ g) {
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) st_sys(&g[acc_row(x, r) * 64 + acc_col(y)], t.c[x][y][r]);
-}
-
 __device__ __forceinline__ void lds_load_sys(double* __restrict__ s, const double* __restrict__ g) {
   const int c = threadIdx.x & 63;
 #pragma unroll 4

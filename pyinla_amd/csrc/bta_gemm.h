@@ -38,6 +38,14 @@ struct GemmJob {
   int lower;     // 1: only tiles with tr >= tc
   int mirror;    // with lower: also write the transpose into the upper triangle
   int nseg;
+  // split = 2 (lower jobs of the 64-tile kernel only): two workgroups per
+  // output tile, one per half of the tile's K chunks; each writes its partial
+  // sum to part (two 64 x 64 slots per tile) and counts itself in cnt (one
+  // counter per tile, zero between launches), and the second to arrive adds
+  // the other's partial and writes the tile.  Set by DevPlan::build.
+  int split;
+  double* part;
+  int* cnt;
   GemmSeg seg[3];
 };
 
@@ -53,7 +61,7 @@ struct GemmLaunch {
     if (prefix.empty()) prefix.push_back(0);
     const int tm = (j.m + ts - 1) / ts, tn = (j.n + ts - 1) / ts;
     jobs.push_back(j);
-    prefix.push_back(prefix.back() + (j.lower ? tm * (tm + 1) / 2 : tm * tn));
+    prefix.push_back(prefix.back() + (j.lower ? tm * (tm + 1) / 2 : tm * tn) * j.split);
   }
 };
 
@@ -94,6 +102,9 @@ inline GemmJob job(double* C, int m, int n, int ldc, double beta, int lower = 0,
   j.lower = lower;
   j.mirror = mirror;
   j.nseg = 0;
+  j.split = 1;
+  j.part = nullptr;
+  j.cnt = nullptr;
   return j;
 }
 
@@ -129,10 +140,14 @@ struct GemvJob {
 struct DevPlan {
   // vec: every operand has an even leading dimension and a 16-byte aligned
   // base (the pipelined kernel's paired loads need both)
-  struct Entry { int job_off, pre_off, njobs, tiles, ts, vec; };
+  // split: the launch holds split jobs (run by the 64-tile kernel whatever
+  // the variant asked for)
+  struct Entry { int job_off, pre_off, njobs, tiles, ts, vec, split; };
   GemmJob* d_jobs = nullptr;
   int* d_prefix = nullptr;
-  size_t bytes = 0;  // device bytes of the job table and prefix sums
+  double* d_part = nullptr;  // partial sums of split jobs, shared by the launches (one stream)
+  int* d_cnt = nullptr;      // their per-tile counters
+  size_t bytes = 0;  // device bytes of the job table, prefix sums and split scratch
   std::vector<Entry> launches;
   DevPlan() = default;
   DevPlan(const DevPlan&) = delete;

@@ -1,4 +1,5 @@
 // Grouped FP64 GEMM and GEMV kernels + device job-table plans.
+#include <algorithm>
 #include <stdexcept>
 
 #include "bta_gemm.h"
@@ -126,9 +127,12 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
                                                              const int* __restrict__ prefix, int njobs) {
   __shared__ double sA[64 * BTA_GEMM_LD];
   __shared__ double sB[64 * BTA_GEMM_LD];
+  __shared__ int last_sh;
   const int jid = find_job(prefix, njobs, blockIdx.x);
   const GemmJob& g = jobs[jid];
-  const int local = blockIdx.x - prefix[jid];
+  const bool split = g.split == 2;
+  const int local = split ? (blockIdx.x - prefix[jid]) >> 1 : blockIdx.x - prefix[jid];
+  const int half = split ? (blockIdx.x - prefix[jid]) & 1 : 0;
   int tr, tc;
   if (g.lower) decode_lower(local, tr, tc);
   else {
@@ -148,7 +152,7 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
   const int wrows = min(64, g.m - r0), wcols = min(64, g.n - c0);
   double* ct = g.C + (int64_t)r0 * g.ldc + c0;
   TileAcc acc;
-  if (g.beta != 0.0) {
+  if (g.beta != 0.0 && half == 0) {  // the second half of a split tile starts from zero
     acc_load(acc, ct, g.ldc, wrows, wcols);
     if (g.beta != 1.0) {
 #pragma unroll
@@ -200,7 +204,23 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
     if (sg.tb == 1) pb.load(sg.B + (int64_t)c0 * sg.ldb + k0, sg.ldb, 0, wcols, kw, mb, c0, k0);
     else pb.load(sg.B + (int64_t)k0 * sg.ldb + c0, sg.ldb, 1, wcols, kw, mb, c0, k0);
   };
+  // a split tile's halves take the first ceil(nch / 2) and the remaining of
+  // its nch chunks
+  int qend = 0x7fffffff;
+  int q = 0;
+  if (split) {
+    int nch = 0;
+    for (int sid = 0; sid < g.nseg; ++sid) {
+      int kb, ke;
+      seg_range(sid, kb, ke);
+      if (ke > kb) nch += (ke - kb + BTA_GEMM_KC - 1) / BTA_GEMM_KC;
+    }
+    qend = half ? nch : (nch + 1) / 2;
+  }
   bool have = (g.nseg > 0) && next_chunk(true);
+  if (half)
+    for (const int q0 = qend - qend / 2; have && q < q0; ++q) have = next_chunk(false);
+  if (q >= qend) have = false;
   if (have) fetch();
   while (have) {
     const GemmSeg& sg = g.seg[seg_id];
@@ -208,10 +228,36 @@ __global__ void __launch_bounds__(BTA_THREADS, BTA_GEMM_MINB) k_gemm_grouped(con
     pa.store(sA, sg.ta, sg.alpha);
     pb.store(sB, sg.tb == 1 ? 0 : 1);
     __syncthreads();
-    have = next_chunk(false);
+    have = next_chunk(false) && ++q < qend;
     if (have) fetch();
     mma_nt_ld<BTA_GEMM_LD>(acc, sA, sB, (kw + 3) & ~3);
     lds_barrier();  // not a wait for the next chunk's loads: its stores wait for them
+  }
+  if (split) {
+    // both halves publish their partial sums (write-through stores that have
+    // completed before the counter moves); the second to count adds the other
+    // half's partial (a + b == b + a: the tile is the same whichever comes
+    // second), resets the counter for the next launch and writes the tile
+    double* mine = g.part + ((int64_t)local * 2 + half) * 4096;
+    acc_store_sys(acc, mine);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int before = __hip_atomic_fetch_add(g.cnt + local, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      if (before == 1) __hip_atomic_store(g.cnt + local, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      last_sh = before;
+    }
+    __syncthreads();
+    if (last_sh != 1) return;
+    const double* other = g.part + ((int64_t)local * 2 + (half ^ 1)) * 4096;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc.c[x][y][r] += ld_sys(&other[acc_row(x, r) * 64 + acc_col(y)]);
   }
   // epilogue
 #pragma unroll
@@ -803,8 +849,12 @@ DevPlan::~DevPlan() { release(); }
 void DevPlan::release() {
   if (d_jobs) hipFree(d_jobs);
   if (d_prefix) hipFree(d_prefix);
+  if (d_part) hipFree(d_part);
+  if (d_cnt) hipFree(d_cnt);
   d_jobs = nullptr;
   d_prefix = nullptr;
+  d_part = nullptr;
+  d_cnt = nullptr;
   bytes = 0;
   launches.clear();
 }
@@ -813,6 +863,10 @@ int DevPlan::build(const std::vector<GemmLaunch>& ls) {
   release();
   std::vector<GemmJob> all;
   std::vector<int> pre;
+  // split jobs: tile offsets into one scratch shared by all launches (they run
+  // one after another on the plan's stream), sized for the largest launch
+  int64_t split_max = 0;
+  std::vector<int64_t> split_off;
   for (const auto& l : ls) {
     Entry e;
     e.job_off = (int)all.size();
@@ -821,33 +875,58 @@ int DevPlan::build(const std::vector<GemmLaunch>& ls) {
     e.tiles = l.tiles();
     e.ts = l.ts;
     e.vec = 1;
-    for (const GemmJob& j : l.jobs)
+    e.split = 0;
+    int64_t split_tiles = 0;
+    for (const GemmJob& j : l.jobs) {
       for (int q = 0; q < j.nseg; ++q) {
         const GemmSeg& sg = j.seg[q];
         if ((sg.lda & 1) || (sg.ldb & 1) || (reinterpret_cast<uintptr_t>(sg.A) & 15) ||
             (reinterpret_cast<uintptr_t>(sg.B) & 15))
           e.vec = 0;
       }
+      if (j.split != 1 && (j.split != 2 || !j.lower || l.ts != 64)) return (int)hipErrorInvalidValue;
+      split_off.push_back(j.split == 2 ? split_tiles : -1);
+      if (j.split == 2) {
+        const int tm = (j.m + 63) / 64;
+        split_tiles += tm * (tm + 1) / 2;
+        e.split = 1;
+      }
+    }
+    split_max = std::max(split_max, split_tiles);
     launches.push_back(e);
     all.insert(all.end(), l.jobs.begin(), l.jobs.end());
     pre.insert(pre.end(), l.prefix.begin(), l.prefix.end());
   }
   if (all.empty()) return 0;
   hipError_t e;
+  if (split_max > 0) {
+    if ((e = hipMalloc(&d_part, (size_t)split_max * 2 * 4096 * sizeof(double))) != hipSuccess) return (int)e;
+    if ((e = hipMalloc(&d_cnt, (size_t)split_max * sizeof(int))) != hipSuccess) return (int)e;
+    if ((e = hipMemset(d_cnt, 0, (size_t)split_max * sizeof(int))) != hipSuccess) return (int)e;
+    for (size_t q = 0; q < all.size(); ++q)
+      if (split_off[q] >= 0) {
+        all[q].part = d_part + split_off[q] * 2 * 4096;
+        all[q].cnt = d_cnt + split_off[q];
+      }
+  }
   if ((e = hipMalloc(&d_jobs, all.size() * sizeof(GemmJob))) != hipSuccess) return (int)e;
   if ((e = hipMalloc(&d_prefix, pre.size() * sizeof(int))) != hipSuccess) return (int)e;
   if ((e = hipMemcpy(d_jobs, all.data(), all.size() * sizeof(GemmJob), hipMemcpyHostToDevice)) != hipSuccess)
     return (int)e;
   if ((e = hipMemcpy(d_prefix, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
     return (int)e;
-  bytes = all.size() * sizeof(GemmJob) + pre.size() * sizeof(int);
+  bytes = all.size() * sizeof(GemmJob) + pre.size() * sizeof(int) +
+          (size_t)split_max * (2 * 4096 * sizeof(double) + sizeof(int));
   return 0;
 }
 
 void DevPlan::launch(int idx, hipStream_t st, int variant) const {
   const Entry& e = launches[idx];
   if (e.tiles <= 0) return;
-  if (e.ts == BIG_TS)
+  if (e.split)
+    hipLaunchKernelGGL(k_gemm_grouped, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
+                       d_prefix + e.pre_off, e.njobs);
+  else if (e.ts == BIG_TS)
     hipLaunchKernelGGL(k_gemm_grouped_big, dim3(e.tiles), dim3(BTA_THREADS), 0, st, d_jobs + e.job_off,
                        d_prefix + e.pre_off, e.njobs);
   else if (variant == 1 && e.vec)

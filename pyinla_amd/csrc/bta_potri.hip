@@ -26,6 +26,10 @@
 #include "bta_internal.h"
 #include "../../include/pyinla_bta.h"
 
+#ifndef BTA_SELINV_SPLIT
+#define BTA_SELINV_SPLIT 1
+#endif
+
 namespace bta {
 
 int potri_build(Workspace& ws, PotriPlan& pp, double* D, double* L, double* B, double* T, int n, int b, int a);
@@ -126,6 +130,10 @@ int potri_build(Workspace& ws, PotriPlan& pp, double* D, double* L, double* B, d
   double* Gaall = ws.gfull + (size_t)n * b * b;
   std::vector<GemmLaunch> ls;
   const int ts = choose_tile(ws.tile_grouped, b, b);
+  // Sigma_ii's update (lower-triangle tiles, K = b) as split tiles: alone it
+  // would fill only about half of the resident slots with workgroups that each
+  // run the whole K loop
+  const int split_sd = (BTA_SELINV_SPLIT && ts == 64 && b >= 1024) ? 2 : 1;
   // One launch of everything the recurrence does not depend on:
   //   G_i = L_i W_i, Ga_i = B_i W_i, Sigma_ii <- W_i^T W_i (the first term of
   //   Sigma_ii; D_i is free once W_i exists), Sigma_T = T^-T T^-1.
@@ -192,6 +200,7 @@ int potri_build(Workspace& ws, PotriPlan& pp, double* D, double* L, double* B, d
     if (a > 0 && i > 0) l3.add(arrow_job(i - 1));
     if (hasL || a > 0) {
       GemmJob sd = job(Di, b, b, b, 1.0, /*lower=*/1, /*mirror=*/1);  // Sigma_ii
+      sd.split = split_sd;
       if (hasL) add_seg(sd, seg(G, b, 1, Li, b, 0, b, -1.0));
       if (a > 0) add_seg(sd, seg(Ga, b, 1, Bi, b, 0, a, -1.0));
       l3.add(sd);
@@ -239,6 +248,7 @@ int potri_build(Workspace& ws, PotriPlan& pp, double* D, double* L, double* B, d
       }
       if (hasL || a > 0) {
         GemmJob sd = job(Di, b, b, b, 1.0, /*lower=*/1, /*mirror=*/1);  // Sigma_ii
+        sd.split = split_sd;
         if (hasL) add_seg(sd, seg(Gall + (int64_t)i * b * b, b, 1, Li, b, 0, b, -1.0));
         if (a > 0) add_seg(sd, seg(Ga, b, 1, Bi, b, 0, a, -1.0));
         l3.add(sd);

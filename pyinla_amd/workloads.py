@@ -36,7 +36,7 @@ class KnownFactorBTA:
     rhs: np.ndarray
 
 
-def known_factor_draws(rng: np.random.Generator, n: int, b: int, a: int):
+def known_factor_draws(rng: np.random.Generator, n: int, b: int, a: int, norm_b: int | None = None):
     """Draw the factor blocks in a fixed order.
 
     Draw order: (1) diag-block strict-lower entries N(0, 0.1), shape (n, b, b);
@@ -44,13 +44,16 @@ def known_factor_draws(rng: np.random.Generator, n: int, b: int, a: int):
     N(0, 0.1), shape (n-1, b, b); (4) arrow blocks N(0, 0.05), shape (n, a, b);
     (5) tip strict-lower N(0, 0.05), shape (a, a); (6) tip diagonal
     1 + |N(0,1)|, shape (a,); (7) rhs N(0,1), shape (n*b + a,).
+    ``norm_b`` scales the drawn (1), (3) and (4) by sqrt(norm_b / b), as
+    :func:`known_factor_bta_torch` does (the draws themselves are unchanged).
     """
-    Ld = np.tril(rng.normal(0.0, 0.1, (n, b, b)), -1)
+    sc = (norm_b / b) ** 0.5 if norm_b else 1.0
+    Ld = np.tril(rng.normal(0.0, 0.1, (n, b, b)), -1) * sc
     dg = np.abs(rng.normal(0.0, 1.0, (n, b))) + 1.0
     idx = np.arange(b)
     Ld[:, idx, idx] = dg
-    Ll = rng.normal(0.0, 0.1, (max(n - 1, 0), b, b))
-    Lb = rng.normal(0.0, 0.05, (n, a, b))
+    Ll = rng.normal(0.0, 0.1, (max(n - 1, 0), b, b)) * sc
+    Lb = rng.normal(0.0, 0.05, (n, a, b)) * sc
     Lt = np.tril(rng.normal(0.0, 0.05, (a, a)), -1)
     Lt[np.arange(a), np.arange(a)] = 1.0 + np.abs(rng.normal(0.0, 1.0, a))
     rhs = rng.normal(0.0, 1.0, n * b + a)
@@ -81,13 +84,17 @@ def q_from_factor(Ld, Ll, Lb, Lt):
     return QD, QL, QB, QT
 
 
-def known_factor_bta(n: int = 50, b: int = 92, a: int = 4, seed: int = 0) -> KnownFactorBTA:
+def known_factor_bta(n: int = 50, b: int = 92, a: int = 4, seed: int = 0,
+                     norm_b: int | None = None) -> KnownFactorBTA:
     """Plain BTA test matrix: n diagonal blocks of size b, arrowhead a.
 
     Defaults are the plain FP64 BTA solve case (n=50, b=92, a=4, seed 0).
+    ``norm_b``: off-diagonal draws scaled to the block norms of the b = norm_b
+    construction (see :func:`known_factor_bta_torch`: unscaled, large b gives a
+    badly conditioned Q).
     """
     rng = np.random.default_rng(seed)
-    Ld, Ll, Lb, Lt, rhs = known_factor_draws(rng, n, b, a)
+    Ld, Ll, Lb, Lt, rhs = known_factor_draws(rng, n, b, a, norm_b)
     QD, QL, QB, QT = q_from_factor(Ld, Ll, Lb, Lt)
     return KnownFactorBTA(n, b, a, Ld, Ll, Lb, Lt, QD, QL, QB, QT, rhs)
 

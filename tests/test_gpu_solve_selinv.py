@@ -23,13 +23,21 @@ def _rel(x, y):
 
 
 CASES = [(1, 1, 0), (1, 40, 3), (2, 70, 2), (3, 17, 3), (3, 64, 4), (4, 92, 4), (2, 130, 0), (3, 200, 9),
-         (50, 92, 4), (3, 100, 64), (3, 600, 5), (2, 640, 0)]  # b >= 512 exercises the 128x128-tile GEMM
+         (50, 92, 4), (3, 100, 64), (3, 600, 5), (2, 640, 0),  # b >= 512 exercises the 128x128-tile GEMM
+         (3, 1100, 5), (2, 1024, 0)]  # b >= 1024: Sigma_ii's update as split tiles (1100: ragged edge tiles)
+
+
+def _kf(n, b, a, seed):
+    # b >= 1024: off-diagonal draws scaled to the b = 92 block norms (unscaled,
+    # Sigma spans ~7 orders of magnitude at n = 3 and the 1e-9 bar measures the
+    # conditioning, not the kernels)
+    return known_factor_bta(n, b, a, seed=seed, norm_b=92 if b >= 1024 else None)
 
 
 @pytest.mark.parametrize("n,b,a", CASES)
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_solve_matches_oracle(cuda, n, b, a, use_graph):
-    kf = known_factor_bta(n, b, a, seed=100 + n * b + a)
+    kf = _kf(n, b, a, seed=100 + n * b + a)
     s = SerinvSolver(None, b, n, a, use_graph=use_graph)
     s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if a else None, kf.Q_T if a else None)
     s.factorize("bta")
@@ -51,7 +59,7 @@ def test_solve_matches_oracle(cuda, n, b, a, use_graph):
 
 @pytest.mark.parametrize("n,b,a", CASES)
 def test_selected_inversion_matches_oracle(cuda, n, b, a):
-    kf = known_factor_bta(n, b, a, seed=200 + n * b + a)
+    kf = _kf(n, b, a, seed=200 + n * b + a)
     s = SerinvSolver(None, b, n, a, use_graph=False)
     s.load_blocks(kf.Q_D, kf.Q_L, kf.Q_B if a else None, kf.Q_T if a else None)
     s.factorize("bta")
@@ -198,13 +206,13 @@ def test_batched_flow_factorization(cuda, force_flow):
 
 @pytest.mark.parametrize("use_graph", [False, True])
 @pytest.mark.parametrize("blas", [False, True])
-@pytest.mark.parametrize("n,b,a", [(2, 70, 2), (4, 92, 4), (3, 130, 0), (3, 300, 6)])
+@pytest.mark.parametrize("n,b,a", [(2, 70, 2), (4, 92, 4), (3, 130, 0), (3, 300, 6), (3, 1100, 5)])
 def test_selected_inversion_blas_and_grouped(cuda, n, b, a, blas, use_graph):
     """Both forms of the Sigma_{i+1,i} products (rocBLAS dgemm / grouped
     kernel) against the oracle, twice in a row on the same solver; with
     use_graph the second call replays the recorded graph, which holds the
     all-grouped plan (rocBLAS is not called under capture)."""
-    kf = known_factor_bta(n, b, a, seed=300 + n * b + a)
+    kf = _kf(n, b, a, seed=300 + n * b + a)
     s = SerinvSolver(None, b, n, a, use_graph=use_graph)
     s.set_selinv_blas(blas)
     D, L = kf.Lf_D.copy(), kf.Lf_L.copy()
