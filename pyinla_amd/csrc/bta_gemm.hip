@@ -13,6 +13,9 @@ namespace bta {
 #define BTA_GEMM_KC 32
 #endif
 #define BTA_GEMM_LD (BTA_GEMM_KC + 1)  // odd stride: conflict-free 16-row fragment reads
+#ifndef BTA_GEMM_PAIRS
+#define BTA_GEMM_PAIRS 1  // 16-byte loads of interior chunks where aligned
+#endif
 #ifndef BTA_GEMM_MINB
 #define BTA_GEMM_MINB 4  // resident workgroups per CU the register budget is cut for
 #endif
@@ -80,45 +83,66 @@ __device__ __forceinline__ void decode_lower(int idx, int& tr, int& tc) {
 }
 
 // A 64 x BTA_GEMM_KC operand chunk staged global -> registers -> LDS (same
-// conventions as TilePrefetch: LDS layout s[i][k] of op(X)).
+// conventions as TilePrefetch: LDS layout s[i][k] of op(X)).  Thread t holds
+// pairs of contiguous elements: r[2q + u] is (i, k) = (t/16 + 16q, 2(t%16) + u)
+// of a row-major chunk (trans 0) or (k, i) = (t/32 + 8q, 2(t%32) + u) of a
+// transposed one (trans 1), so an interior chunk with an even leading
+// dimension and a 16-byte aligned origin is fetched with 16-byte loads.
 struct ChunkPrefetch {
+  static_assert(BTA_GEMM_KC == 32 && BTA_THREADS == 256, "pair mapping assumes 32-wide chunks, 256 threads");
   static constexpr int N = 64 * BTA_GEMM_KC / BTA_THREADS;
   double r[N];
   __device__ __forceinline__ void load(const double* __restrict__ g0, int64_t ld, int trans, int rows, int kw,
                                        int mask, int i0, int k0) {
     const gdouble* g = gview(g0);
     mask = chunk_mask(mask, i0, rows, k0, kw);
+    const int64_t step = trans == 0 ? 16 * ld : 8 * ld;
+    const gdouble* t = trans == 0 ? g + (int64_t)(threadIdx.x >> 4) * ld + 2 * (threadIdx.x & 15)
+                                  : g + (int64_t)(threadIdx.x >> 5) * ld + 2 * (threadIdx.x & 31);
     if (mask == MASK_NONE && rows >= 64 && kw >= BTA_GEMM_KC) {  // interior chunk: no tests
-      if (trans == 0) {
-        const gdouble* t = g + (int64_t)(threadIdx.x / BTA_GEMM_KC) * ld + threadIdx.x % BTA_GEMM_KC;
+      if (BTA_GEMM_PAIRS && (ld & 1) == 0 && (reinterpret_cast<uintptr_t>(g0) & 15) == 0) {
 #pragma unroll
-        for (int q = 0; q < N; ++q) r[q] = t[(int64_t)q * (BTA_THREADS / BTA_GEMM_KC) * ld];
+        for (int q = 0; q < N / 2; ++q) {
+          const f64x2 x = *(const gf64x2*)(t + q * step);
+          r[2 * q] = x.x;
+          r[2 * q + 1] = x.y;
+        }
       } else {
-        const gdouble* t = g + (int64_t)(threadIdx.x >> 6) * ld + (threadIdx.x & 63);
 #pragma unroll
-        for (int q = 0; q < N; ++q) r[q] = t[(int64_t)q * (BTA_THREADS / 64) * ld];
+        for (int q = 0; q < N / 2; ++q) {
+          r[2 * q] = t[q * step];
+          r[2 * q + 1] = t[q * step + 1];
+        }
       }
       return;
     }
 #pragma unroll
-    for (int q = 0; q < N; ++q) {
-      const int e = threadIdx.x + BTA_THREADS * q;
-      if (trans == 0) {
-        const int i = e / BTA_GEMM_KC, k = e % BTA_GEMM_KC;
-        r[q] = (i < rows && k < kw && !masked_out(mask, i0 + i, k0 + k)) ? g[(int64_t)i * ld + k] : 0.0;
-      } else {
-        const int k = e >> 6, i = e & 63;
-        r[q] = (k < kw && i < rows && !masked_out(mask, i0 + i, k0 + k)) ? g[(int64_t)k * ld + i] : 0.0;
+    for (int q = 0; q < N / 2; ++q)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        int i, k;
+        if (trans == 0) {
+          i = (threadIdx.x >> 4) + 16 * q;
+          k = 2 * (threadIdx.x & 15) + u;
+        } else {
+          k = (threadIdx.x >> 5) + 8 * q;
+          i = 2 * (threadIdx.x & 31) + u;
+        }
+        r[2 * q + u] = (i < rows && k < kw && !masked_out(mask, i0 + i, k0 + k)) ? t[q * step + u] : 0.0;
       }
-    }
   }
   __device__ __forceinline__ void store(double* __restrict__ s, int trans, double alpha = 1.0) const {
 #pragma unroll
-    for (int q = 0; q < N; ++q) {
-      const int e = threadIdx.x + BTA_THREADS * q;
-      const double v = alpha * r[q];
-      if (trans == 0) s[(e / BTA_GEMM_KC) * BTA_GEMM_LD + (e % BTA_GEMM_KC)] = v;
-      else s[(e & 63) * BTA_GEMM_LD + (e >> 6)] = v;
+    for (int q = 0; q < N / 2; ++q) {
+      if (trans == 0) {
+        const int i = (threadIdx.x >> 4) + 16 * q, k = 2 * (threadIdx.x & 15);
+        s[i * BTA_GEMM_LD + k] = alpha * r[2 * q];
+        s[i * BTA_GEMM_LD + k + 1] = alpha * r[2 * q + 1];
+      } else {
+        const int k = (threadIdx.x >> 5) + 8 * q, i = 2 * (threadIdx.x & 31);
+        s[i * BTA_GEMM_LD + k] = alpha * r[2 * q];
+        s[(i + 1) * BTA_GEMM_LD + k] = alpha * r[2 * q + 1];
+      }
     }
   }
 };
