@@ -84,20 +84,23 @@ __device__ __forceinline__ void decode_lower(int idx, int& tr, int& tc) {
 
 // A 64 x BTA_GEMM_KC operand chunk staged global -> registers -> LDS (same
 // conventions as TilePrefetch: LDS layout s[i][k] of op(X)).  Thread t holds
-// pairs of contiguous elements: r[2q + u] is (i, k) = (t/16 + 16q, 2(t%16) + u)
-// of a row-major chunk (trans 0) or (k, i) = (t/32 + 8q, 2(t%32) + u) of a
-// transposed one (trans 1), so an interior chunk with an even leading
-// dimension and a 16-byte aligned origin is fetched with 16-byte loads.
+// pairs of contiguous elements: with H = KC / 2 threads per chunk row,
+// r[2q + u] is (i, k) = (t / H + (256 / H) q, 2 (t % H) + u) of a row-major
+// chunk (trans 0) or (k, i) = (t / 32 + 8q, 2 (t % 32) + u) of a transposed
+// one (trans 1), so an interior chunk with an even leading dimension and a
+// 16-byte aligned origin is fetched with 16-byte loads.
 struct ChunkPrefetch {
-  static_assert(BTA_GEMM_KC == 32 && BTA_THREADS == 256, "pair mapping assumes 32-wide chunks, 256 threads");
+  static_assert(BTA_THREADS == 256 && BTA_GEMM_KC % 8 == 0 && BTA_GEMM_KC <= 64, "pair mapping");
   static constexpr int N = 64 * BTA_GEMM_KC / BTA_THREADS;
+  static constexpr int H = BTA_GEMM_KC / 2;     // threads per row of a row-major chunk
+  static constexpr int RQ = BTA_THREADS / H;    // its rows per pair step
   double r[N];
   __device__ __forceinline__ void load(const double* __restrict__ g0, int64_t ld, int trans, int rows, int kw,
                                        int mask, int i0, int k0) {
     const gdouble* g = gview(g0);
     mask = chunk_mask(mask, i0, rows, k0, kw);
-    const int64_t step = trans == 0 ? 16 * ld : 8 * ld;
-    const gdouble* t = trans == 0 ? g + (int64_t)(threadIdx.x >> 4) * ld + 2 * (threadIdx.x & 15)
+    const int64_t step = trans == 0 ? RQ * ld : 8 * ld;
+    const gdouble* t = trans == 0 ? g + (int64_t)(threadIdx.x / H) * ld + 2 * (threadIdx.x % H)
                                   : g + (int64_t)(threadIdx.x >> 5) * ld + 2 * (threadIdx.x & 31);
     if (mask == MASK_NONE && rows >= 64 && kw >= BTA_GEMM_KC) {  // interior chunk: no tests
       if (BTA_GEMM_PAIRS && (ld & 1) == 0 && (reinterpret_cast<uintptr_t>(g0) & 15) == 0) {
@@ -122,8 +125,8 @@ struct ChunkPrefetch {
       for (int u = 0; u < 2; ++u) {
         int i, k;
         if (trans == 0) {
-          i = (threadIdx.x >> 4) + 16 * q;
-          k = 2 * (threadIdx.x & 15) + u;
+          i = threadIdx.x / H + RQ * q;
+          k = 2 * (threadIdx.x % H) + u;
         } else {
           k = (threadIdx.x >> 5) + 8 * q;
           i = 2 * (threadIdx.x & 31) + u;
@@ -135,7 +138,7 @@ struct ChunkPrefetch {
 #pragma unroll
     for (int q = 0; q < N / 2; ++q) {
       if (trans == 0) {
-        const int i = (threadIdx.x >> 4) + 16 * q, k = 2 * (threadIdx.x & 15);
+        const int i = threadIdx.x / H + RQ * q, k = 2 * (threadIdx.x % H);
         s[i * BTA_GEMM_LD + k] = alpha * r[2 * q];
         s[i * BTA_GEMM_LD + k + 1] = alpha * r[2 * q + 1];
       } else {
